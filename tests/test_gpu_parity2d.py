@@ -1,0 +1,89 @@
+"""GPU parity of the 2D hot path (4 variables, 4x4 Newton con2prim) against the CPU oracle."""
+import numpy as np
+import pytest
+
+from helpers import from_oracle2, interior, per_var_rel_l1, progress, rel_l1, state2, to_oracle2
+
+pytestmark = pytest.mark.gpu
+
+G = 4.0 / 3.0
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def grid():
+    from verona_amd import DeviceGrid
+    dev = DeviceGrid(2, 72, 56, 1, G)
+    yield dev
+    dev.close()
+
+
+@pytest.mark.parametrize("rec", ["wenoz", "ppm", "minmod"])
+@pytest.mark.parametrize("riemann", ["hll", "hllc"])
+def test_fluxes2(grid, oracle, rec, riemann):
+    progress(f"test_fluxes2 rec={rec} riemann={riemann}")
+    P = state2(72, 56, seed=1)
+    grid.upload("P", P)
+    Po = to_oracle2(P)
+    for dim, slot in enumerate(["Fx", "Fy"]):
+        grid.fluxes(dim, 1e-7, rec, riemann)
+        F = grid.download(slot)
+        Fo = from_oracle2(oracle.fluxes2(Po, G, 1e-7, dim, rec, riemann))
+        sl = (slice(None), slice(2, -3), slice(2, -3))
+        assert rel_l1(F[sl], Fo[sl]) <= TOL
+
+
+def test_update_c2p2(grid, oracle):
+    progress("test_update_c2p2")
+    P = state2(72, 56, seed=2)
+    Po = to_oracle2(P)
+    dx, dy = 1 / 72, 1 / 56
+    dt = 0.4 / (1 / dx + 1 / dy)
+    grid.upload("P", P)
+    grid.prim_to_cons()
+    grid.fluxes(0, 1e-7)
+    grid.fluxes(1, 1e-7)
+    grid.update("Uhalf", dt / 2, dx, dy)
+    grid.cons_to_prim("Uhalf", 50, 1e-10)
+    grid.limit(1e-7)
+    Pg = grid.download("P")
+    Uo = oracle.ptou2(Po, G)
+    Fx = oracle.fluxes2(Po, G, 1e-7, 0)
+    Fy = oracle.fluxes2(Po, G, 1e-7, 1)
+    Uho = oracle.update2(Uo, dt / 2, dx, dy, Fx, Fy)
+    oracle.c2p2(Uho, Po, G, 50, 1e-10)
+    oracle.limit2(Po, 1e-7)
+    errs = per_var_rel_l1(Pg, from_oracle2(Po))
+    progress(f"  per-var rel L1 {errs}")
+    assert max(errs) <= TOL
+
+
+@pytest.mark.parametrize("mode", ["ref", "ext"])
+def test_fused_step2(oracle, mode):
+    from verona_amd import DeviceGrid
+    progress(f"test_fused_step2 {mode}")
+    nx, ny = 72, 56
+    P = state2(nx, ny, seed=3)
+    dx, dy = 1 / nx, 1 / ny
+    ref = mode == "ref"
+    cfl = 0.0 if ref else 0.4
+    dt = 0.4 / (1 / dx + 1 / dy) if ref else 0.0
+    integ = "rk2" if ref else "rk3"
+    riem = "hll" if ref else "hllc"
+    bc = ["periodic"] * 4 + ["none", "none"]
+    with DeviceGrid(2, nx, ny, 1, G) as dev:
+        dev.upload("P", P)
+        dev.fill_ghosts(bc)
+        dev.prim_to_cons()
+        dev.set_step(ref_mode=ref, rec="wenoz", riemann=riem, integrator=integ, n_iter=50,
+                     tol=1e-10, floor=1e-7, dt=dt, cfl=cfl, dx=dx, dy=dy, bc=bc)
+        for _ in range(3):
+            dev.step()
+        Pg = dev.download("P")
+    Po = to_oracle2(P)
+    m = oracle.make_mode(ref=ref, rec="wenoz", riemann=riem, integrator=integ, n_iter=50,
+                         tol=1e-10, gamma=G, floor=1e-7, dt=dt, cfl=cfl, dx=dx, dy=dy)
+    oracle.run2(Po, m, [1, 1, 1, 1], 3)
+    errs = per_var_rel_l1(interior(Pg), interior(from_oracle2(Po)))
+    progress(f"  per-var rel L1 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= TOL

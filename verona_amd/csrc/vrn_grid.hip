@@ -1,0 +1,1412 @@
+// vrn_grid.hip -- 2D/3D grid kernels of the SRHD hot path for gfx950, and
+// the C ABI of include/verona_hip.h.
+//
+// Device layout: structure of arrays, one plane per variable.  Cell (i, j, k)
+// (0-based, ghosts included) of variable v lives at
+//     v * ps + XOFF + i + px * (j + NY * k)
+// with a row pitch px that is a multiple of 16 doubles and XOFF = 13, so the
+// first interior cell of every row (i = 3) sits on a 128-byte boundary and a
+// 64-lane wave reading interior cells of a row touches exactly four lines.
+//
+// Two kernel sets:
+//  * the reference boundary (vrn_fluxes / vrn_update / vrn_cons_to_prim /
+//    vrn_limit / vrn_prim_to_cons): one thread per face or cell, same
+//    arrays and regions as the Julia KA kernels they replace;
+//  * the fused step (vrn_step): per RK stage, one directional sweep kernel
+//    per axis.  The x sweep runs 64 lanes along a row as a continuous stream
+//    of cells over many rows and passes the upper face state and the face
+//    flux to the next lane with a cross-lane shift; the y and z sweeps give
+//    each lane one (x, .) column and march along the axis with the 5-cell
+//    reconstruction window held in registers.  Either way each cell's
+//    reconstruction and each face's Riemann solve run exactly once, and all
+//    global loads and stores are row-contiguous across the wave.  The last
+//    sweep of a stage also forms the RK combination, runs 1DW con2prim +
+//    Limit, and (CFL mode) reduces the signal speed; cells 1DW leaves
+//    unconverged go to a rescue list finished by the full cascade.
+#include "../../include/verona_hip.h"
+#include "vrn_physics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace vrn {
+
+constexpr int XOFF = 13;
+
+struct Geo {
+    int NX, NY, NZ;  // with ghosts; NZ = 1 in 2D
+    int px;          // row pitch in doubles
+    long ps;         // plane stride in doubles
+};
+
+__host__ __device__ __forceinline__ long gidx(const Geo &G, int i, int j, int k)
+{
+    return XOFF + i + (long)G.px * (j + (long)G.NY * k);
+}
+
+struct DevState {
+    double dt;       // dt of the current step (CFL mode)
+    double dt_prev;  // dt of the step that just finished
+    unsigned long long rate_bits;   // max signal rate (non-negative double bits)
+    unsigned int rescue_count;
+    unsigned int pad0;
+    unsigned long long n_unrecovered;
+    unsigned long long n_rescued;
+    unsigned long long n_singular;
+};
+
+struct SweepArgs {
+    Geo geo;
+    const double *P;   // primitives of this stage (read)
+    const double *U0;  // U^n (last sweep, a != 0)
+    const double *Ub;  // base state (first sweep)
+    double *Uacc;      // accumulator (first: write; middle: read/write; last: read)
+    double *Uout;      // last sweep: combined conserved state
+    double *Pout;      // last sweep: recovered primitives
+    DevState *st;
+    int *rescue;
+    double g, floor_;
+    double inv_dx[3];
+    double dtfac;     // stage dt = dt * dtfac
+    double dx_dim;    // spacing along this sweep's axis
+    double dt_fixed;  // dt when dt_dev == 0
+    int dt_dev;       // 1: dt comes from DevState (CFL mode)
+    double a, b;    // stage output = a U0 + b acc (a == 0: acc)
+    double tol;
+    int n_iter;
+    int do_rate;
+    int seg_len;
+    int rows_per_wave;
+};
+
+// ------------------------------------------------------------ wave helpers
+template <class T> __device__ __forceinline__ T shift_up1(T v) { return __shfl_up(v, 1); }
+template <class T> __device__ __forceinline__ T lane63(T v) { return __shfl(v, 63); }
+
+__device__ __forceinline__ double wave_max(double r)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off));
+    return r;
+}
+
+__device__ __forceinline__ void rate_reduce(DevState *st, double r)
+{
+    if (!(r >= 0.0)) r = 0.0;  // NaN rates are ignored, as in a `r > max` scan
+    r = wave_max(r);
+    if (__lane_id() == 0) atomicMax(&st->rate_bits, (unsigned long long)__double_as_longlong(r));
+}
+
+__device__ __forceinline__ void rescue_push(int *list, DevState *st, bool need, int idx)
+{
+    unsigned long long m = __ballot(need);
+    if (m == 0) return;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0;
+    if (__lane_id() == leader) base = atomicAdd(&st->rescue_count, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    if (need) {
+        unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
+        list[base + __popcll(below)] = idx;
+    }
+}
+
+template <int NV> __device__ __forceinline__ bool all_finite(const double *p)
+{
+    bool ok = true;
+#pragma unroll
+    for (int v = 0; v < NV; v++) ok = ok && __builtin_isfinite(p[v]);
+    return ok;
+}
+
+// Combine, con2prim, Limit, rate for one updated cell of the last sweep.
+// A cell counts as unrecovered when its con2prim result is not finite (the
+// reference's NaN marker, or a NaN from a branch that reported convergence).
+template <int NV>
+__device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const double *acc,
+                                            const double *u0, const double *guess, bool valid,
+                                            double &rate)
+{
+    const long ps = a.geo.ps;
+    double out[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = (a.a == 0.0) ? acc[v] : a.a * u0[v] + a.b * acc[v];
+    bool need = false;
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) a.Uout[v * ps + cu] = out[v];
+        double Pn[NV];
+        int iters = 0;
+        bool ok;
+        if constexpr (NV == 5) {
+            ok = c2p_1dw(out, guess, a.g, a.n_iter, a.tol, Pn, iters);
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) Pn[v] = guess[v];
+            ok = c2p_2d(out, Pn, a.g, a.n_iter, a.tol, iters);
+            if (!ok) atomicAdd(&a.st->n_singular, 1ull);
+            ok = true;
+        }
+        if (ok) {
+            if (!all_finite<NV>(Pn)) atomicAdd(&a.st->n_unrecovered, 1ull);
+            Pn[0] = jmax(Pn[0], a.floor_);
+            Pn[1] = jmax(Pn[1], a.floor_);
+#pragma unroll
+            for (int v = 0; v < NV; v++) a.Pout[v * ps + cu] = Pn[v];
+            if (a.do_rate) rate = fmax(rate, cell_rate<NV>(Pn, a.g, a.inv_dx));
+        } else {
+            need = true;
+        }
+    }
+    if constexpr (NV == 5) rescue_push(a.rescue, a.st, need, (int)cu);
+}
+
+// ------------------------------------------------------------ x sweep
+// Lanes walk a stream of (row, cell) pairs: rows are interior (j, k), cells
+// c = 2 .. NX-3 hold the reconstructions; lane with cell c also solves face
+// c-1/2 (c >= 3) and updates cell c-1 (c >= 4).
+template <int NV, class R, int REC, int RIEM, bool LAST>
+__global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
+{
+    const Geo G = a.geo;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int Nyi = G.NY - 6;
+    const int Nzi = NV == 5 ? G.NZ - 6 : 1;
+    const int nrows = Nyi * Nzi;
+    const int r0 = wave * a.rows_per_wave;
+    if (r0 >= nrows) return;
+    const int r1 = min(r0 + a.rows_per_wave, nrows);
+    const int E = G.NX - 4;
+    const int total = (r1 - r0) * E;
+    const long ps = G.ps;
+    const R g = (R)a.g, fl = (R)a.floor_;
+    const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
+    const double *__restrict__ P = a.P;
+
+    R cup[NV];
+    double cF[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        cup[v] = R(0);
+        cF[v] = 0.0;
+    }
+    double rate = 0.0;
+    for (int s0 = 0; s0 < total; s0 += 64) {
+        const int s = s0 + lane;
+        const bool act = s < total;
+        const int ss = act ? s : total - 1;
+        const int r = r0 + ss / E;
+        const int c = 2 + ss % E;
+        const int j = 3 + r % Nyi;
+        const int k = NV == 5 ? 3 + r / Nyi : 0;
+        const long rb = gidx(G, 0, j, k);
+        R lo[NV], up[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double *q = P + v * ps + rb + c;
+            recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
+        }
+        lo[0] = jmax(fl, lo[0]);
+        lo[1] = jmax(fl, lo[1]);
+        up[0] = jmax(fl, up[0]);
+        up[1] = jmax(fl, up[1]);
+        R upl[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            R t = shift_up1(up[v]);
+            upl[v] = lane == 0 ? cup[v] : t;
+        }
+        R Ff[NV];
+        face_flux<NV, 0, RIEM>(upl, lo, g, Ff);
+        double F[NV], Fl[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            F[v] = (double)Ff[v];
+            double t = shift_up1(F[v]);
+            Fl[v] = lane == 0 ? cF[v] : t;
+        }
+        const bool upd = act && c >= 4;
+        const long cu = rb + (c - 1);
+        if constexpr (!LAST) {
+            if (upd) {
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    a.Uacc[v * ps + cu] = a.Ub[v * ps + cu] - cdt * (F[v] - Fl[v]);
+            }
+        } else {
+            // 2D: the x sweep is never last; kept for completeness of the template.
+            double acc[NV], u0[NV], guess[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                acc[v] = upd ? a.Ub[v * ps + cu] - cdt * (F[v] - Fl[v]) : 0.0;
+                u0[v] = (upd && a.a != 0.0) ? a.U0[v * ps + cu] : 0.0;
+                guess[v] = upd ? P[v * ps + cu] : 1.0;
+            }
+            finish_cell<NV>(a, cu, acc, u0, guess, upd, rate);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            cup[v] = lane63(up[v]);
+            cF[v] = lane63(F[v]);
+        }
+    }
+    if constexpr (LAST)
+        if (a.do_rate) rate_reduce(a.st, rate);
+}
+
+// ------------------------------------------------------------ y / z sweeps
+// One lane per interior (x, o) column, marching along DIM over the segment
+// [m0, m1) of interior cells with the 5-cell window in registers.
+template <int NV, int DIM, class R, int REC, int RIEM, bool LAST>
+__global__ __launch_bounds__(256) void k_march(SweepArgs a)
+{
+    const Geo G = a.geo;
+    const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int o = 3 + blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int Nd = DIM == 1 ? G.NY : G.NZ;
+    const int No = NV == 5 ? (DIM == 1 ? G.NZ : G.NY) : 7;
+    const int m0 = 3 + blockIdx.z * a.seg_len;
+    // whole-wave exits only: a partially active wave keeps all lanes for the
+    // cross-lane reductions and masks the inactive ones.
+    if (o >= No - 3 || m0 >= Nd - 3) return;
+    const bool act = x < G.NX - 3;
+    const int xx = act ? x : 3;
+    const int m1 = min(m0 + a.seg_len, Nd - 3);
+    const long sd = DIM == 1 ? (long)G.px : (long)G.px * G.NY;
+    const long cb = DIM == 1 ? gidx(G, xx, 0, NV == 5 ? o : 0) : gidx(G, xx, o, 0);
+    const long ps = G.ps;
+    const R g = (R)a.g, fl = (R)a.floor_;
+    const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
+    const double *__restrict__ P = a.P;
+    const double *Ui = a.Uacc;
+
+    R w0[NV], w1[NV], w2[NV], w3[NV], w4[NV];
+    double nx[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double *q = P + v * ps + cb;
+        w0[v] = (R)q[(m0 - 3) * sd];
+        w1[v] = (R)q[(m0 - 2) * sd];
+        w2[v] = (R)q[(m0 - 1) * sd];
+        w3[v] = (R)q[(m0)*sd];
+        nx[v] = q[(m0 + 1) * sd];
+    }
+    R upp[NV];
+    double Fp[NV], ub[NV], u0b[NV], gb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        upp[v] = R(0);
+        Fp[v] = 0.0;
+        ub[v] = 0.0;
+        u0b[v] = 0.0;
+        gb[v] = 0.0;
+    }
+    double rate = 0.0;
+    for (int c = m0 - 1; c <= m1; c++) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) w4[v] = (R)nx[v];
+        if (c + 3 < Nd) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) nx[v] = P[v * ps + cb + (c + 3) * sd];
+        }
+        // prefetch the conserved state of cell c, updated at iteration c+1
+        double un[NV], u0n[NV], gn[NV];
+        const bool pre = c >= m0 && c < m1;
+        const long cc = cb + (long)c * sd;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            un[v] = pre ? Ui[v * ps + cc] : 0.0;
+            u0n[v] = (LAST && pre && a.a != 0.0) ? a.U0[v * ps + cc] : 0.0;
+            if constexpr (LAST && !std::is_same<R, double>::value) gn[v] = pre ? P[v * ps + cc] : 0.0;
+        }
+        R lo[NV], up[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) recon<REC>(w0[v], w1[v], w2[v], w3[v], w4[v], lo[v], up[v]);
+        lo[0] = jmax(fl, lo[0]);
+        lo[1] = jmax(fl, lo[1]);
+        up[0] = jmax(fl, up[0]);
+        up[1] = jmax(fl, up[1]);
+        if (c >= m0) {
+            R Ff[NV];
+            face_flux<NV, DIM, RIEM>(upp, lo, g, Ff);
+            double F[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) F[v] = (double)Ff[v];
+            if (c - 1 >= m0) {
+                const long cu = cb + (long)(c - 1) * sd;
+                double acc[NV];
+#pragma unroll
+                for (int v = 0; v < NV; v++) acc[v] = ub[v] - cdt * (F[v] - Fp[v]);
+                if constexpr (!LAST) {
+                    if (act) {
+#pragma unroll
+                        for (int v = 0; v < NV; v++) a.Uacc[v * ps + cu] = acc[v];
+                    }
+                } else {
+                    double guess[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        if constexpr (std::is_same<R, double>::value)
+                            guess[v] = w1[v];
+                        else
+                            guess[v] = gb[v];
+                    }
+                    finish_cell<NV>(a, cu, acc, u0b, guess, act, rate);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NV; v++) Fp[v] = F[v];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            upp[v] = up[v];
+            ub[v] = un[v];
+            u0b[v] = u0n[v];
+            if constexpr (LAST && !std::is_same<R, double>::value) gb[v] = gn[v];
+            w0[v] = w1[v];
+            w1[v] = w2[v];
+            w2[v] = w3[v];
+            w3[v] = w4[v];
+        }
+    }
+    if constexpr (LAST)
+        if (a.do_rate) rate_reduce(a.st, rate);
+}
+
+// ------------------------------------------------------------ rescue (3D)
+// Cells whose 1DW Newton did not converge: rest of the cascade from the
+// stage's guess, then Limit.  The list holds linear device indices.
+__global__ __launch_bounds__(64) void k_rescue(SweepArgs a)
+{
+    const unsigned int n = a.st->rescue_count;
+    const long ps = a.geo.ps;
+    double rate = 0.0;
+    for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const long cu = a.rescue[t];
+        double U[5], Pn[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            U[v] = a.Uout[v * ps + cu];
+            Pn[v] = a.P[v * ps + cu];
+        }
+        int iters = 0;
+        c2p_cascade_tail(U, Pn, a.g, a.n_iter, a.tol, iters);
+        if (!all_finite<5>(Pn)) atomicAdd(&a.st->n_unrecovered, 1ull);
+        Pn[0] = jmax(Pn[0], a.floor_);
+        Pn[1] = jmax(Pn[1], a.floor_);
+#pragma unroll
+        for (int v = 0; v < 5; v++) a.Pout[v * ps + cu] = Pn[v];
+        if (a.do_rate) rate = fmax(rate, cell_rate<5>(Pn, a.g, a.inv_dx));
+    }
+    if (a.do_rate) rate_reduce(a.st, rate);
+}
+
+// ------------------------------------------------------------ ghosts
+// One thread per ghost cell.  The value written is the source cell reached
+// by mapping each ghost coordinate through its face's rule (periodic wrap or
+// zero-gradient copy; identity for faces without a rule), which is what
+// applying whole-plane passes x, then y, then z produces.  A source cell
+// that is interior along every axis comes from Pi, otherwise from Pg (with
+// the floor when do_limit).  Sources are never ghosts of a mapped face, so
+// the in-place form (Pd == Pi == Pg) is race free.
+struct GhostArgs {
+    Geo geo;
+    double *Pd;
+    const double *Pi;
+    const double *Pg;
+    int rule_lo[3], rule_hi[3];  // 0 identity, 1 periodic, 2 outflow
+    int do_limit;
+    double floor_;
+    int nozzle;
+    double noz_r2, noz_dx, noz_dy, noz_rho, noz_u, noz_ux, noz_uy, noz_uz;
+    DevState *st;
+};
+
+__device__ __forceinline__ int map_coord(int c, int N, int lo, int hi)
+{
+    if (c < 3) return lo == 1 ? c + (N - 6) : (lo == 2 ? 3 : c);
+    if (c >= N - 3) return hi == 1 ? c - (N - 6) : (hi == 2 ? N - 4 : c);
+    return c;
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_ghost(GhostArgs a)
+{
+    const Geo G = a.geo;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && a.st != nullptr) {
+        a.st->n_rescued += a.st->rescue_count;
+        a.st->rescue_count = 0;
+    }
+    int i, j, k;
+    if constexpr (NV == 5) {
+        const long nz6 = 6L * G.NX * G.NY;
+        const long ny6 = 6L * G.NX * (G.NZ - 6);
+        const long nx6 = 6L * (G.NY - 6) * (G.NZ - 6);
+        if (t < nz6) {
+            long r = t;
+            i = (int)(r % G.NX);
+            r /= G.NX;
+            j = (int)(r % G.NY);
+            int q = (int)(r / G.NY);
+            k = q < 3 ? q : G.NZ - 6 + q;
+        } else if (t < nz6 + ny6) {
+            long r = t - nz6;
+            i = (int)(r % G.NX);
+            r /= G.NX;
+            int q = (int)(r % 6);
+            j = q < 3 ? q : G.NY - 6 + q;
+            k = 3 + (int)(r / 6);
+        } else if (t < nz6 + ny6 + nx6) {
+            long r = t - nz6 - ny6;
+            int q = (int)(r % 6);
+            i = q < 3 ? q : G.NX - 6 + q;
+            r /= 6;
+            j = 3 + (int)(r % (G.NY - 6));
+            k = 3 + (int)(r / (G.NY - 6));
+        } else {
+            return;
+        }
+    } else {
+        const long ny6 = 6L * G.NX;
+        const long nx6 = 6L * (G.NY - 6);
+        k = 0;
+        if (t < ny6) {
+            i = (int)(t % G.NX);
+            int q = (int)(t / G.NX);
+            j = q < 3 ? q : G.NY - 6 + q;
+        } else if (t < ny6 + nx6) {
+            long r = t - ny6;
+            int q = (int)(r % 6);
+            i = q < 3 ? q : G.NX - 6 + q;
+            j = 3 + (int)(r / 6);
+        } else {
+            return;
+        }
+    }
+    const long ps = G.ps;
+    const long dst = gidx(G, i, j, k);
+    if (NV == 5 && a.nozzle && k < 3 && i >= 3 && i < G.NX - 3 && j >= 3 && j < G.NY - 3) {
+        double xc = ((i - 3) + 0.5) * a.noz_dx - 0.5 * (G.NX - 6) * a.noz_dx;
+        double yc = ((j - 3) + 0.5) * a.noz_dy - 0.5 * (G.NY - 6) * a.noz_dy;
+        if (xc * xc + yc * yc < a.noz_r2) {
+            a.Pd[dst] = a.noz_rho;
+            a.Pd[ps + dst] = a.noz_u;
+            a.Pd[2 * ps + dst] = a.noz_ux;
+            a.Pd[3 * ps + dst] = a.noz_uy;
+            a.Pd[4 * ps + dst] = a.noz_uz;
+            return;
+        }
+    }
+    const int mi = map_coord(i, G.NX, a.rule_lo[0], a.rule_hi[0]);
+    const int mj = map_coord(j, G.NY, a.rule_lo[1], a.rule_hi[1]);
+    const int mk = NV == 5 ? map_coord(k, G.NZ, a.rule_lo[2], a.rule_hi[2]) : 0;
+    const bool interior = mi >= 3 && mi < G.NX - 3 && mj >= 3 && mj < G.NY - 3 &&
+                          (NV == 4 || (mk >= 3 && mk < G.NZ - 3));
+    const long src = gidx(G, mi, mj, mk);
+    if (interior) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) a.Pd[v * ps + dst] = a.Pi[v * ps + src];
+    } else {
+        double p[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) p[v] = a.Pg[v * ps + src];
+        if (a.do_limit) {
+            p[0] = jmax(p[0], a.floor_);
+            p[1] = jmax(p[1], a.floor_);
+        }
+        if (src != dst || a.do_limit || a.Pd != a.Pg) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) a.Pd[v * ps + dst] = p[v];
+        }
+    }
+}
+
+// ------------------------------------------------------------ CFL
+template <int NV>
+__global__ __launch_bounds__(256) void k_rate(Geo G, const double *P, double g, double ix,
+                                               double iy, double iz, DevState *st)
+{
+    const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 3 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = NV == 5 ? 3 + blockIdx.z : 0;
+    if (j >= G.NY - 3) return;
+    double r = 0.0;
+    if (x < G.NX - 3) {
+        const long c = gidx(G, x, j, k);
+        double p[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) p[v] = P[v * G.ps + c];
+        const double inv[3] = {ix, iy, iz};
+        r = cell_rate<NV>(p, g, inv);
+    }
+    rate_reduce(st, r);
+}
+
+__global__ void k_finalize_dt(DevState *st, double cfl)
+{
+    double r = __longlong_as_double((long long)st->rate_bits);
+    st->dt_prev = st->dt;
+    st->dt = cfl / r;
+    st->rate_bits = 0ull;
+}
+
+// ------------------------------------------------------------ reference-boundary kernels
+template <int NV>
+__global__ __launch_bounds__(256) void k_ref_ptou(Geo G, const double *P, double *U, double g)
+{
+    const long n = (long)G.NX * G.NY * G.NZ;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % G.NX);
+        const long r = t / G.NX;
+        const int j = (int)(r % G.NY), k = (int)(r / G.NY);
+        const long c = gidx(G, i, j, k);
+        double p[NV], u[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) p[v] = P[v * G.ps + c];
+        p2u<NV>(p, u, g, lorentz<NV>(p));
+#pragma unroll
+        for (int v = 0; v < NV; v++) U[v * G.ps + c] = u[v];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_ref_limit(Geo G, double *P, double fl)
+{
+    const long n = (long)G.NX * G.NY * G.NZ;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % G.NX);
+        const long r = t / G.NX;
+        const long c = gidx(G, i, (int)(r % G.NY), (int)(r / G.NY));
+        P[c] = jmax(P[c], fl);
+        P[G.ps + c] = jmax(P[G.ps + c], fl);
+    }
+}
+
+// Fluxes: face i+1/2 along DIM for 0-based (i, j, k) in [2, N-3)^ndim; both
+// face states recomputed per face as in the reference (two reconstructions).
+template <int NV, int DIM, int REC, int RIEM>
+__global__ __launch_bounds__(256) void k_ref_fluxes(Geo G, const double *P, double *F, double g,
+                                                     double fl)
+{
+    const int ni = G.NX - 5, nj = G.NY - 5, nk = NV == 5 ? G.NZ - 5 : 1;
+    const long n = (long)ni * nj * nk;
+    const long sd = DIM == 0 ? 1 : (DIM == 1 ? (long)G.px : (long)G.px * G.NY);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = 2 + (int)(t % ni);
+        const long r = t / ni;
+        const int j = 2 + (int)(r % nj);
+        const int k = NV == 5 ? 2 + (int)(r / nj) : 0;
+        const long c = gidx(G, i, j, k);
+        double PL[NV], PR[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double *q = P + v * G.ps + c;
+            double lo, hi;
+            recon<REC>(q[-2 * sd], q[-sd], q[0], q[sd], q[2 * sd], lo, hi);
+            PL[v] = hi;
+            recon<REC>(q[-sd], q[0], q[sd], q[2 * sd], q[3 * sd], lo, hi);
+            PR[v] = lo;
+        }
+        PL[0] = jmax(fl, PL[0]);
+        PR[0] = jmax(fl, PR[0]);
+        PL[1] = jmax(fl, PL[1]);
+        PR[1] = jmax(fl, PR[1]);
+        double Fo[NV];
+        face_flux<NV, DIM, RIEM>(PL, PR, g, Fo);
+#pragma unroll
+        for (int v = 0; v < NV; v++) F[v * G.ps + c] = Fo[v];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_ref_update(Geo G, const double *U, double *Ub, double dt,
+                                                     double dx, double dy, double dz,
+                                                     const double *Fx, const double *Fy,
+                                                     const double *Fz)
+{
+    const int ni = G.NX - 6, nj = G.NY - 6, nk = NV == 5 ? G.NZ - 6 : 1;
+    const long n = (long)ni * nj * nk;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = 3 + (int)(t % ni);
+        const long r = t / ni;
+        const int j = 3 + (int)(r % nj);
+        const int k = NV == 5 ? 3 + (int)(r / nj) : 0;
+        const long c = gidx(G, i, j, k);
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const long cv = v * G.ps + c;
+            double val = U[cv] - dt / dx * (Fx[cv] - Fx[cv - 1]) - dt / dy * (Fy[cv] - Fy[cv - G.px]);
+            if constexpr (NV == 5) val = val - dt / dz * (Fz[cv] - Fz[cv - (long)G.px * G.NY]);
+            Ub[cv] = val;
+        }
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(64) void k_ref_c2p(Geo G, const double *U, double *P, double g,
+                                                 int n_iter, double tol, DevState *st)
+{
+    const int ni = G.NX - 6, nj = G.NY - 6, nk = NV == 5 ? G.NZ - 6 : 1;
+    const long n = (long)ni * nj * nk;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = 3 + (int)(t % ni);
+        const long r = t / ni;
+        const int j = 3 + (int)(r % nj);
+        const int k = NV == 5 ? 3 + (int)(r / nj) : 0;
+        const long c = gidx(G, i, j, k);
+        double u[NV], p[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            u[v] = U[v * G.ps + c];
+            p[v] = P[v * G.ps + c];
+        }
+        int iters = 0;
+        if constexpr (NV == 5) {
+            double pn[5];
+            if (c2p_1dw(u, p, g, n_iter, tol, pn, iters)) {
+#pragma unroll
+                for (int v = 0; v < 5; v++) p[v] = pn[v];
+            } else {
+                atomicAdd(&st->n_rescued, 1ull);
+                c2p_cascade_tail(u, p, g, n_iter, tol, iters);
+            }
+        } else {
+            if (!c2p_2d(u, p, g, n_iter, tol, iters)) atomicAdd(&st->n_singular, 1ull);
+        }
+        if (!all_finite<NV>(p)) atomicAdd(&st->n_unrecovered, 1ull);
+#pragma unroll
+        for (int v = 0; v < NV; v++) P[v * G.ps + c] = p[v];
+    }
+}
+
+// AoS (Verona order) <-> SoA planes
+template <int NV>
+__global__ __launch_bounds__(256) void k_aos_to_soa(Geo G, const double *aos, double *soa)
+{
+    const long n = (long)G.NX * G.NY * G.NZ;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % G.NX);
+        const long r = t / G.NX;
+        const long c = gidx(G, i, (int)(r % G.NY), (int)(r / G.NY));
+#pragma unroll
+        for (int v = 0; v < NV; v++) soa[v * G.ps + c] = aos[t * NV + v];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_soa_to_aos(Geo G, const double *soa, double *aos)
+{
+    const long n = (long)G.NX * G.NY * G.NZ;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % G.NX);
+        const long r = t / G.NX;
+        const long c = gidx(G, i, (int)(r % G.NY), (int)(r / G.NY));
+#pragma unroll
+        for (int v = 0; v < NV; v++) aos[t * NV + v] = soa[v * G.ps + c];
+    }
+}
+
+}  // namespace vrn
+
+// ====================================================================== host side
+using namespace vrn;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(VRN_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+const char *kKernelNames[] = {"xsweep", "ysweep", "zsweep_c2p", "ysweep_c2p", "rescue",
+                              "ghost",  "cfl",    "fluxes",     "update",     "c2p"};
+enum { KN_X, KN_Y, KN_ZC, KN_YC, KN_RESCUE, KN_GHOST, KN_CFL, KN_FLUX, KN_UPD, KN_C2P, KN_COUNT };
+
+}  // namespace
+
+struct vrn_ctx {
+    int ndim = 3, NV = 5;
+    Geo G{};
+    double gamma = 4.0 / 3.0;
+    double *buf = nullptr;  // one allocation for all planes
+    double *P = nullptr, *Pn = nullptr, *U = nullptr, *Uh = nullptr, *F[3] = {nullptr, nullptr, nullptr};
+    double *stg = nullptr;
+    int *rescue = nullptr;
+    DevState *st = nullptr;
+    DevState *st_host = nullptr;  // pinned
+    hipStream_t s = nullptr;
+    // nozzle
+    double noz_r = 0, noz_dx = 1, noz_dy = 1, noz_rho = 0, noz_u = 0, noz_umag = 0, noz_eps = 0;
+    bool have_noz = false;
+    bool first_cfl = true;
+    double dxs[3] = {1, 1, 1};
+    // tuning
+    int rows_per_wave = 8;
+    int seg_len = 96;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
+    double t_ms[KN_COUNT] = {0};
+    long t_n[KN_COUNT] = {0};
+};
+
+namespace {
+
+int harvest(vrn_ctx *c)
+{
+    if (c->ev_used.empty()) return 0;
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (auto &e : c->ev_used) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+        c->t_ms[e.first] += ms;
+        c->t_n[e.first] += 1;
+        c->ev_free.push_back(e.second);
+    }
+    c->ev_used.clear();
+    return 0;
+}
+
+// Event bracket around one launch when timing is on.
+struct Timed {
+    vrn_ctx *c;
+    int kn;
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    bool on;
+    Timed(vrn_ctx *c_, int kn_) : c(c_), kn(kn_), on(c_->timing)
+    {
+        if (!on) return;
+        if (c->ev_free.empty()) {
+            if (c->ev_used.size() >= 8192) harvest(c);
+        }
+        if (c->ev_free.empty()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+                on = false;
+                return;
+            }
+            c->ev_free.push_back({a, b});
+        }
+        ev = c->ev_free.back();
+        c->ev_free.pop_back();
+        hipEventRecord(ev.first, c->s);
+    }
+    ~Timed()
+    {
+        if (!on) return;
+        hipEventRecord(ev.second, c->s);
+        c->ev_used.push_back({kn, ev});
+    }
+};
+
+double *slot_ptr(vrn_ctx *c, int slot)
+{
+    switch (slot) {
+    case VRN_SLOT_P: return c->P;
+    case VRN_SLOT_U: return c->U;
+    case VRN_SLOT_UHALF: return c->Uh;
+    case VRN_SLOT_FX: return c->F[0];
+    case VRN_SLOT_FY: return c->F[1];
+    case VRN_SLOT_FZ: return c->ndim == 3 ? c->F[2] : nullptr;
+    default: return nullptr;
+    }
+}
+
+int grid_blocks(long n, int block)
+{
+    long b = (n + block - 1) / block;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+template <int NV> void launch_ghost(vrn_ctx *c, GhostArgs &ga)
+{
+    const Geo &G = c->G;
+    long n;
+    if (NV == 5)
+        n = 6L * G.NX * G.NY + 6L * G.NX * (G.NZ - 6) + 6L * (G.NY - 6) * (G.NZ - 6);
+    else
+        n = 6L * G.NX + 6L * (G.NY - 6);
+    Timed tm(c, KN_GHOST);
+    hipLaunchKernelGGL(k_ghost<NV>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s, ga);
+}
+
+// rules: per face 0 identity, 1 periodic, 2 outflow (nozzle counts as outflow)
+void make_rules(const int bc[6], bool periodic_only, bool nonperiodic_only, int *lo, int *hi,
+                int &nozzle)
+{
+    nozzle = 0;
+    for (int d = 0; d < 3; d++) {
+        for (int s = 0; s < 2; s++) {
+            int b = bc[2 * d + s];
+            int r = 0;
+            if (b == VRN_BC_PERIODIC && !nonperiodic_only) r = 1;
+            if ((b == VRN_BC_OUTFLOW || b == VRN_BC_NOZZLE) && !periodic_only) r = 2;
+            (s == 0 ? lo : hi)[d] = r;
+        }
+    }
+    if (bc[4] == VRN_BC_NOZZLE && !periodic_only) nozzle = 1;
+}
+
+void fill_nozzle_args(vrn_ctx *c, GhostArgs &ga, double phi)
+{
+    ga.noz_r2 = c->noz_r * c->noz_r;
+    ga.noz_dx = c->noz_dx;
+    ga.noz_dy = c->noz_dy;
+    ga.noz_rho = c->noz_rho;
+    ga.noz_u = c->noz_u;
+    double nx = c->noz_eps * cos(phi), ny = c->noz_eps * sin(phi), nz = 1.0;
+    double nn = sqrt(nx * nx + ny * ny + nz * nz);
+    ga.noz_ux = c->noz_umag * (nx / nn);
+    ga.noz_uy = c->noz_umag * (ny / nn);
+    ga.noz_uz = c->noz_umag * (nz / nn);
+}
+
+int check_ctx(vrn_ctx *c)
+{
+    if (!c) return fail(VRN_E_ARG, "null context");
+    return 0;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *vrn_last_error(void) { return g_err.c_str(); }
+
+int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
+{
+    if (!out) return fail(VRN_E_ARG, "out is null");
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3) return fail(VRN_E_ARG, "ndim must be 2 or 3 (got %d)", ndim);
+    if (nx < 4 || ny < 4 || (ndim == 3 && nz < 4))
+        return fail(VRN_E_ARG, "interior sizes must be >= 4 (got %d %d %d)", nx, ny, nz);
+    if (!(gamma > 1.0)) return fail(VRN_E_ARG, "gamma must be > 1");
+    vrn_ctx *c = new vrn_ctx();
+    c->ndim = ndim;
+    c->NV = ndim == 3 ? 5 : 4;
+    c->gamma = gamma;
+    c->G.NX = nx + 6;
+    c->G.NY = ny + 6;
+    c->G.NZ = ndim == 3 ? nz + 6 : 1;
+    c->G.px = ((XOFF + c->G.NX + 15) / 16) * 16;
+    c->G.ps = (long)c->G.px * c->G.NY * c->G.NZ;
+    long cells = (long)c->G.NX * c->G.NY * c->G.NZ;
+    if ((long)c->G.ps >= (1L << 31)) {
+        delete c;
+        return fail(VRN_E_ARG, "grid too large for 32-bit rescue indices");
+    }
+    const int NV = c->NV;
+    int nplanes_arrays = ndim == 3 ? 7 : 6;  // P, Pn, U, Uh, Fx, Fy, (Fz)
+    size_t arr = (size_t)NV * c->G.ps;
+    size_t total = arr * nplanes_arrays + (size_t)NV * cells;
+    hipError_t e = hipStreamCreate(&c->s);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(VRN_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    e = hipMalloc(&c->buf, total * sizeof(double));
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->s);
+        delete c;
+        return fail(VRN_E_NOMEM, "hipMalloc(%zu bytes): %s", total * sizeof(double), hipGetErrorString(e));
+    }
+    double *p = c->buf;
+    c->P = p; p += arr;
+    c->Pn = p; p += arr;
+    c->U = p; p += arr;
+    c->Uh = p; p += arr;
+    c->F[0] = p; p += arr;
+    c->F[1] = p; p += arr;
+    if (ndim == 3) { c->F[2] = p; p += arr; }
+    c->stg = p;
+    e = hipMalloc(&c->rescue, sizeof(int) * (size_t)cells);
+    if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(DevState));
+    if (e == hipSuccess) e = hipHostMalloc(&c->st_host, sizeof(DevState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemsetAsync(c->buf, 0, total * sizeof(double), c->s);
+    if (e == hipSuccess) e = hipMemsetAsync(c->st, 0, sizeof(DevState), c->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+    if (e != hipSuccess) {
+        vrn_destroy(c);
+        return fail(VRN_E_NOMEM, "allocation failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return 0;
+}
+
+int vrn_destroy(vrn_ctx *c)
+{
+    if (!c) return 0;
+    if (c->s) hipStreamSynchronize(c->s);
+    for (auto &e : c->ev_used) {
+        hipEventDestroy(e.second.first);
+        hipEventDestroy(e.second.second);
+    }
+    for (auto &e : c->ev_free) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    if (c->buf) hipFree(c->buf);
+    if (c->rescue) hipFree(c->rescue);
+    if (c->st) hipFree(c->st);
+    if (c->st_host) hipHostFree(c->st_host);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+    return 0;
+}
+
+int vrn_sync(vrn_ctx *c)
+{
+    if (int r = check_ctx(c)) return r;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int vrn_upload(vrn_ctx *c, int slot, const double *host)
+{
+    if (int r = check_ctx(c)) return r;
+    double *d = slot_ptr(c, slot);
+    if (!d || !host) return fail(VRN_E_ARG, "bad slot %d or null host pointer", slot);
+    const Geo &G = c->G;
+    size_t bytes = sizeof(double) * (size_t)c->NV * G.NX * G.NY * G.NZ;
+    HIPCHK(hipMemcpyAsync(c->stg, host, bytes, hipMemcpyHostToDevice, c->s));
+    long n = (long)G.NX * G.NY * G.NZ;
+    if (c->NV == 5)
+        hipLaunchKernelGGL(k_aos_to_soa<5>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, c->stg, d);
+    else
+        hipLaunchKernelGGL(k_aos_to_soa<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, c->stg, d);
+    HIPCHK(hipGetLastError());
+    if (slot == VRN_SLOT_P) c->first_cfl = true;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int vrn_download(vrn_ctx *c, int slot, double *host)
+{
+    if (int r = check_ctx(c)) return r;
+    double *d = slot_ptr(c, slot);
+    if (!d || !host) return fail(VRN_E_ARG, "bad slot %d or null host pointer", slot);
+    const Geo &G = c->G;
+    long n = (long)G.NX * G.NY * G.NZ;
+    if (c->NV == 5)
+        hipLaunchKernelGGL(k_soa_to_aos<5>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, d, c->stg);
+    else
+        hipLaunchKernelGGL(k_soa_to_aos<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, d, c->stg);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, c->stg, sizeof(double) * (size_t)c->NV * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int vrn_prim_to_cons(vrn_ctx *c)
+{
+    if (int r = check_ctx(c)) return r;
+    long n = (long)c->G.NX * c->G.NY * c->G.NZ;
+    if (c->NV == 5)
+        hipLaunchKernelGGL(k_ref_ptou<5>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, c->G, c->P, c->U, c->gamma);
+    else
+        hipLaunchKernelGGL(k_ref_ptou<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, c->G, c->P, c->U, c->gamma);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vrn_limit(vrn_ctx *c, double floor_)
+{
+    if (int r = check_ctx(c)) return r;
+    long n = (long)c->G.NX * c->G.NY * c->G.NZ;
+    if (c->NV == 5)
+        hipLaunchKernelGGL(k_ref_limit<5>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, c->G, c->P, floor_);
+    else
+        hipLaunchKernelGGL(k_ref_limit<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, c->G, c->P, floor_);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"
+
+template <int NV, int DIM>
+static void launch_ref_flux(vrn_ctx *c, int rec, int riem, double fl)
+{
+    const Geo &G = c->G;
+    long n = (long)(G.NX - 5) * (G.NY - 5) * (NV == 5 ? G.NZ - 5 : 1);
+    dim3 gb(grid_blocks(n, 256)), bb(256);
+    double *F = c->F[DIM];
+#define VRN_FL(R_, M_) hipLaunchKernelGGL((k_ref_fluxes<NV, DIM, R_, M_>), gb, bb, 0, c->s, G, c->P, F, c->gamma, fl)
+    if (riem == VRN_RIEMANN_HLL) {
+        if (rec == VRN_REC_WENOZ) VRN_FL(REC_WENOZ, RIEM_HLL);
+        else if (rec == VRN_REC_PPM) VRN_FL(REC_PPM, RIEM_HLL);
+        else VRN_FL(REC_MINMOD, RIEM_HLL);
+    } else {
+        if (rec == VRN_REC_WENOZ) VRN_FL(REC_WENOZ, RIEM_HLLC);
+        else if (rec == VRN_REC_PPM) VRN_FL(REC_PPM, RIEM_HLLC);
+        else VRN_FL(REC_MINMOD, RIEM_HLLC);
+    }
+#undef VRN_FL
+}
+
+extern "C" {
+
+int vrn_fluxes(vrn_ctx *c, int dim, double floor_, int rec, int riemann)
+{
+    if (int r = check_ctx(c)) return r;
+    if (dim < 0 || dim >= c->ndim) return fail(VRN_E_ARG, "dim %d out of range", dim);
+    if (rec < 0 || rec > 2 || riemann < 0 || riemann > 1) return fail(VRN_E_ARG, "bad rec/riemann");
+    Timed tm(c, KN_FLUX);
+    if (c->NV == 5) {
+        if (dim == 0) launch_ref_flux<5, 0>(c, rec, riemann, floor_);
+        else if (dim == 1) launch_ref_flux<5, 1>(c, rec, riemann, floor_);
+        else launch_ref_flux<5, 2>(c, rec, riemann, floor_);
+    } else {
+        if (dim == 0) launch_ref_flux<4, 0>(c, rec, riemann, floor_);
+        else launch_ref_flux<4, 1>(c, rec, riemann, floor_);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vrn_update(vrn_ctx *c, int dst_slot, double dt, double dx, double dy, double dz)
+{
+    if (int r = check_ctx(c)) return r;
+    if (dst_slot != VRN_SLOT_U && dst_slot != VRN_SLOT_UHALF) return fail(VRN_E_ARG, "dst must be U or UHALF");
+    double *dst = slot_ptr(c, dst_slot);
+    const Geo &G = c->G;
+    long n = (long)(G.NX - 6) * (G.NY - 6) * (c->NV == 5 ? G.NZ - 6 : 1);
+    Timed tm(c, KN_UPD);
+    if (c->NV == 5)
+        hipLaunchKernelGGL(k_ref_update<5>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, c->U, dst, dt, dx, dy, dz, c->F[0], c->F[1], c->F[2]);
+    else
+        hipLaunchKernelGGL(k_ref_update<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, c->U, dst, dt, dx, dy, dz, c->F[0], c->F[1], c->F[1]);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vrn_cons_to_prim(vrn_ctx *c, int src_slot, int n_iter, double tol)
+{
+    if (int r = check_ctx(c)) return r;
+    if (src_slot != VRN_SLOT_U && src_slot != VRN_SLOT_UHALF) return fail(VRN_E_ARG, "src must be U or UHALF");
+    if (n_iter < 0) return fail(VRN_E_ARG, "n_iter must be >= 0");
+    const Geo &G = c->G;
+    long n = (long)(G.NX - 6) * (G.NY - 6) * (c->NV == 5 ? G.NZ - 6 : 1);
+    Timed tm(c, KN_C2P);
+    if (c->NV == 5)
+        hipLaunchKernelGGL(k_ref_c2p<5>, dim3(grid_blocks(n, 64)), dim3(64), 0, c->s, G, slot_ptr(c, src_slot), c->P, c->gamma, n_iter, tol, c->st);
+    else
+        hipLaunchKernelGGL(k_ref_c2p<4>, dim3(grid_blocks(n, 64)), dim3(64), 0, c->s, G, slot_ptr(c, src_slot), c->P, c->gamma, n_iter, tol, c->st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vrn_set_nozzle(vrn_ctx *c, double radius, double dx, double dy, double rho, double u, double umag, double eps)
+{
+    if (int r = check_ctx(c)) return r;
+    if (c->ndim != 3) return fail(VRN_E_ARG, "nozzle needs a 3D context");
+    c->noz_r = radius;
+    c->noz_dx = dx;
+    c->noz_dy = dy;
+    c->noz_rho = rho;
+    c->noz_u = u;
+    c->noz_umag = umag;
+    c->noz_eps = eps;
+    c->have_noz = true;
+    return 0;
+}
+
+static int fill_ghosts_phi(vrn_ctx *c, const int bc[6], double phi, bool periodic_only, bool nonperiodic_only)
+{
+    GhostArgs ga{};
+    ga.geo = c->G;
+    ga.Pd = c->P;
+    ga.Pi = c->P;
+    ga.Pg = c->P;
+    make_rules(bc, periodic_only, nonperiodic_only, ga.rule_lo, ga.rule_hi, ga.nozzle);
+    if (ga.nozzle) {
+        if (!c->have_noz) return fail(VRN_E_STATE, "VRN_BC_NOZZLE used before vrn_set_nozzle");
+        fill_nozzle_args(c, ga, phi);
+    }
+    ga.do_limit = 0;
+    ga.st = nullptr;
+    if (c->NV == 5) launch_ghost<5>(c, ga);
+    else launch_ghost<4>(c, ga);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vrn_fill_ghosts(vrn_ctx *c, const int bc[6])
+{
+    if (int r = check_ctx(c)) return r;
+    if (!bc) return fail(VRN_E_ARG, "bc is null");
+    for (int f = 0; f < 6; f++)
+        if (bc[f] < 0 || bc[f] > 3 || (bc[f] == VRN_BC_NOZZLE && f != 4))
+            return fail(VRN_E_ARG, "bad boundary kind %d on face %d", bc[f], f);
+    return fill_ghosts_phi(c, bc, 0.0, false, false);
+}
+
+int vrn_get_counters(vrn_ctx *c, int64_t *n_unrecovered, int64_t *n_rescued)
+{
+    if (int r = check_ctx(c)) return r;
+    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (n_unrecovered) *n_unrecovered = (int64_t)c->st_host->n_unrecovered;
+    if (n_rescued) *n_rescued = (int64_t)(c->st_host->n_rescued + c->st_host->rescue_count);
+    return 0;
+}
+
+int vrn_reset_counters(vrn_ctx *c)
+{
+    if (int r = check_ctx(c)) return r;
+    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->st_host->n_unrecovered = 0;
+    c->st_host->n_rescued = 0;
+    c->st_host->n_singular = 0;
+    c->st_host->rescue_count = 0;
+    HIPCHK(hipMemcpyAsync(c->st, c->st_host, sizeof(DevState), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- fused step
+template <int NV, class R, int REC, int RIEM>
+static void launch_stage(vrn_ctx *c, SweepArgs &a)
+{
+    const Geo &G = c->G;
+    // x sweep (first)
+    {
+        SweepArgs ax = a;
+        ax.dx_dim = c->dxs[0];
+        int nrows = (G.NY - 6) * (NV == 5 ? G.NZ - 6 : 1);
+        int waves = (nrows + ax.rows_per_wave - 1) / ax.rows_per_wave;
+        Timed tm(c, KN_X);
+        hipLaunchKernelGGL((k_xsweep<NV, R, REC, RIEM, false>), dim3((waves + 3) / 4), dim3(256), 0, c->s, ax);
+    }
+    const int nxb = (G.NX - 6 + 63) / 64;
+    if constexpr (NV == 5) {
+        {
+            SweepArgs ay = a;
+            ay.dx_dim = c->dxs[1];
+            int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
+            Timed tm(c, KN_Y);
+            hipLaunchKernelGGL((k_march<5, 1, R, REC, RIEM, false>), dim3(nxb, (G.NZ - 6 + 3) / 4, nseg), dim3(256), 0, c->s, ay);
+        }
+        {
+            SweepArgs az = a;
+            az.dx_dim = c->dxs[2];
+            int nseg = (G.NZ - 6 + a.seg_len - 1) / a.seg_len;
+            Timed tm(c, KN_ZC);
+            hipLaunchKernelGGL((k_march<5, 2, R, REC, RIEM, true>), dim3(nxb, (G.NY - 6 + 3) / 4, nseg), dim3(256), 0, c->s, az);
+        }
+        {
+            Timed tm(c, KN_RESCUE);
+            hipLaunchKernelGGL(k_rescue, dim3(256), dim3(64), 0, c->s, a);
+        }
+    } else {
+        SweepArgs ay = a;
+        ay.dx_dim = c->dxs[1];
+        int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
+        Timed tm(c, KN_YC);
+        hipLaunchKernelGGL((k_march<4, 1, R, REC, RIEM, true>), dim3(nxb, 1, nseg), dim3(64), 0, c->s, ay);
+    }
+}
+
+template <int NV>
+static void dispatch_stage(vrn_ctx *c, SweepArgs &a, int rec, int riem, int prec)
+{
+#define VRN_ST(R_, REC_, RI_) launch_stage<NV, R_, REC_, RI_>(c, a)
+#define VRN_REC(R_, RI_)                                   \
+    do {                                                   \
+        if (rec == VRN_REC_WENOZ) VRN_ST(R_, REC_WENOZ, RI_);   \
+        else if (rec == VRN_REC_PPM) VRN_ST(R_, REC_PPM, RI_);  \
+        else VRN_ST(R_, REC_MINMOD, RI_);                  \
+    } while (0)
+    if (prec == VRN_PREC_FP64) {
+        if (riem == VRN_RIEMANN_HLL) VRN_REC(double, RIEM_HLL);
+        else VRN_REC(double, RIEM_HLLC);
+    } else {
+        if (riem == VRN_RIEMANN_HLL) VRN_REC(float, RIEM_HLL);
+        else VRN_REC(float, RIEM_HLLC);
+    }
+#undef VRN_REC
+#undef VRN_ST
+}
+
+extern "C" {
+
+int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
+{
+    if (int r = check_ctx(c)) return r;
+    if (!p) return fail(VRN_E_ARG, "params is null");
+    if (p->rec < 0 || p->rec > 2 || p->riemann < 0 || p->riemann > 1 || p->integrator < 0 ||
+        p->integrator > 1 || p->precision < 0 || p->precision > 1 || p->n_iter < 0)
+        return fail(VRN_E_ARG, "bad step parameters");
+    if (!(p->dx > 0) || !(p->dy > 0) || (c->ndim == 3 && !(p->dz > 0)))
+        return fail(VRN_E_ARG, "grid spacings must be positive");
+    if (!(p->cfl > 0) && !(p->dt > 0)) return fail(VRN_E_ARG, "need dt > 0 or cfl > 0");
+    for (int f = 0; f < 6; f++)
+        if (p->bc[f] < 0 || p->bc[f] > 3 || (p->bc[f] == VRN_BC_NOZZLE && (f != 4 || !c->have_noz)))
+            return fail(VRN_E_ARG, "bad boundary kind %d on face %d (nozzle needs vrn_set_nozzle)", p->bc[f], f);
+    const Geo &G = c->G;
+    const bool cflm = p->cfl > 0.0;
+    // dt: fixed, or CFL from the state at the step start
+    if (cflm) {
+        if (c->first_cfl) {
+            Timed tm(c, KN_CFL);
+            if (c->NV == 5)
+                hipLaunchKernelGGL(k_rate<5>, dim3((G.NX - 6 + 63) / 64, (G.NY - 6 + 3) / 4, G.NZ - 6), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0 / p->dz, c->st);
+            else
+                hipLaunchKernelGGL(k_rate<4>, dim3((G.NX - 6 + 63) / 64, (G.NY - 6 + 3) / 4, 1), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0, c->st);
+            hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->s, c->st, p->cfl);
+            c->first_cfl = false;
+        }
+    }
+    c->dxs[0] = p->dx;
+    c->dxs[1] = p->dy;
+    c->dxs[2] = c->ndim == 3 ? p->dz : 1.0;
+    // Verona sequencing: the user BC callback runs once at the step start
+    if (p->ref_mode) {
+        bool any = false;
+        for (int f = 0; f < 6; f++) any |= (p->bc[f] == VRN_BC_OUTFLOW || p->bc[f] == VRN_BC_NOZZLE);
+        if (any) {
+            int r = fill_ghosts_phi(c, p->bc, p->nozzle_phi, false, true);
+            if (r) return r;
+        }
+    }
+    SweepArgs a{};
+    a.geo = G;
+    a.st = c->st;
+    a.rescue = c->rescue;
+    a.g = c->gamma;
+    a.floor_ = p->floor_;
+    a.inv_dx[0] = 1.0 / p->dx;
+    a.inv_dx[1] = 1.0 / p->dy;
+    a.inv_dx[2] = c->ndim == 3 ? 1.0 / p->dz : 0.0;
+    a.tol = p->tol;
+    a.n_iter = p->n_iter;
+    a.dt_fixed = p->dt;
+    a.dt_dev = cflm ? 1 : 0;
+    a.seg_len = c->seg_len;
+    a.rows_per_wave = c->rows_per_wave;
+    const int nst = p->integrator == VRN_INT_RK3 ? 3 : 2;
+    for (int s = 0; s < nst; s++) {
+        a.P = c->P;
+        a.Pout = c->Pn;
+        a.do_rate = (cflm && s == nst - 1) ? 1 : 0;
+        if (p->integrator == VRN_INT_RK3) {
+            // Shu-Osher SSP-RK3: U1 = U + dt L(U); U2 = 3/4 U + 1/4 (U1 + dt L(U1));
+            // U' = 1/3 U + 2/3 (U2 + dt L(U2))
+            a.dtfac = 1.0;
+            a.U0 = c->U;
+            a.Ub = s == 0 ? c->U : c->Uh;
+            a.Uacc = c->Uh;
+            a.Uout = s == 2 ? c->U : c->Uh;
+            a.a = s == 0 ? 0.0 : (s == 1 ? 0.75 : 1.0 / 3.0);
+            a.b = s == 0 ? 1.0 : (s == 1 ? 0.25 : 2.0 / 3.0);
+        } else {
+            // midpoint RK2 (src/3dim/algos.jl:285, 311): Uhalf = U - dt/2 L(P); U = U - dt L(P*)
+            a.dtfac = s == 0 ? 0.5 : 1.0;
+            a.U0 = c->U;
+            a.Ub = c->U;
+            a.Uacc = s == 0 ? c->Uh : c->U;
+            a.Uout = a.Uacc;
+            a.a = 0.0;
+            a.b = 1.0;
+        }
+        if (c->NV == 5)
+            dispatch_stage<5>(c, a, p->rec, p->riemann, p->precision);
+        else
+            dispatch_stage<4>(c, a, p->rec, p->riemann, p->precision);
+        HIPCHK(hipGetLastError());
+        // ghosts of the new P: carried (floored) from the old one, then the
+        // stage-end boundary rules (ref: periodic exchange only)
+        GhostArgs ga{};
+        ga.geo = G;
+        ga.Pd = c->Pn;
+        ga.Pi = c->Pn;
+        ga.Pg = c->P;
+        make_rules(p->bc, p->ref_mode != 0, false, ga.rule_lo, ga.rule_hi, ga.nozzle);
+        if (ga.nozzle) fill_nozzle_args(c, ga, p->nozzle_phi);
+        ga.do_limit = 1;
+        ga.floor_ = p->floor_;
+        ga.st = c->st;
+        if (c->NV == 5) launch_ghost<5>(c, ga);
+        else launch_ghost<4>(c, ga);
+        HIPCHK(hipGetLastError());
+        std::swap(c->P, c->Pn);
+    }
+    if (cflm) {
+        hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->s, c->st, p->cfl);
+        HIPCHK(hipGetLastError());
+    }
+    if (dt_used) {
+        if (cflm) {
+            // k_finalize_dt moved this step's dt to dt_prev and stored the next one
+            HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            *dt_used = c->st_host->dt_prev;
+        } else {
+            *dt_used = p->dt;
+        }
+    }
+    return 0;
+}
+
+int vrn_timing_enable(vrn_ctx *c, int enable)
+{
+    if (int r = check_ctx(c)) return r;
+    if (int r = harvest(c)) return r;
+    c->timing = enable != 0;
+    for (int k = 0; k < KN_COUNT; k++) {
+        c->t_ms[k] = 0;
+        c->t_n[k] = 0;
+    }
+    return 0;
+}
+
+int vrn_timing_report(vrn_ctx *c, char *names, int name_len, int64_t *counts, double *mean_ms, int cap, int *n_out)
+{
+    if (int r = check_ctx(c)) return r;
+    if (int r = harvest(c)) return r;
+    int n = 0;
+    for (int k = 0; k < KN_COUNT && n < cap; k++) {
+        if (c->t_n[k] == 0) continue;
+        if (names && name_len > 0) {
+            strncpy(names + (size_t)n * name_len, kKernelNames[k], name_len - 1);
+            names[(size_t)n * name_len + name_len - 1] = 0;
+        }
+        if (counts) counts[n] = c->t_n[k];
+        if (mean_ms) mean_ms[n] = c->t_ms[k] / c->t_n[k];
+        n++;
+    }
+    if (n_out) *n_out = n;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,692 @@
+// vrn_physics.hpp -- device point functions of the SRHD hot path (gfx950).
+//
+// Every function follows the operation order of the Julia reference so that
+// an FP64 build with -ffp-contract=off reproduces it to the last bit where
+// the reference itself is IEEE-exact (no contraction, correctly rounded
+// division and square root).  Citations are to Wesenheit/Verona 0.1.0.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace vrn {
+
+// Julia Base max/min for IEEE floats: NaN-propagating, -0 < +0.
+template <class T> __device__ __forceinline__ T jmax(T x, T y)
+{
+    T d = x - y;
+    T am = __builtin_signbit(d) ? y : x;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? d : am;
+}
+template <class T> __device__ __forceinline__ T jmin(T x, T y)
+{
+    T d = x - y;
+    T am = __builtin_signbit(d) ? x : y;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? d : am;
+}
+// Julia sign(): -1, +1, or x itself for +-0 and NaN.
+template <class T> __device__ __forceinline__ T jsign(T x)
+{
+    return x < T(0) ? T(-1) : (x > T(0) ? T(1) : x);
+}
+template <class T> __device__ __forceinline__ T sq(T x) { return x * x; }
+
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+
+enum { REC_MINMOD = 0, REC_PPM = 1, REC_WENOZ = 2 };
+enum { RIEM_HLL = 0, RIEM_HLLC = 1 };
+
+// WENO-Z epsilon: 1e-42 in FP64 (fluxlimiter.jl:56).  1e-42 is subnormal in
+// FP32 and flushes to zero there, so the FP32 form uses 1e-30f instead.
+template <class T> struct WenoEps;
+template <> struct WenoEps<double> { static constexpr double v = 1.0e-42; };
+template <> struct WenoEps<float> { static constexpr float v = 1.0e-30f; };
+
+// fluxlimiter.jl:19-71  WENOZ(q_{i-2..i+2}) -> lo = q at i-1/2, hi = q at i+1/2
+template <class T>
+__device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
+{
+    const T b1c = T(13.0) / T(12.0), b2c = T(0.25);
+    T beta0 = b1c * sq(qm2 + q - T(2) * qm1) + b2c * sq(qm2 + T(3) * q - T(4) * qm1);
+    T beta1 = b1c * sq(qm1 + qp1 - T(2) * q) + b2c * sq(qm1 - qp1);
+    T beta2 = b1c * sq(qp2 + q - T(2) * qp1) + b2c * sq(qp2 + T(3) * q - T(4) * qp1);
+    const T eps = WenoEps<T>::v;
+    T tau5 = dabs(beta0 - beta2);
+    T ind0 = tau5 / (beta0 + eps);
+    T ind1 = tau5 / (beta1 + eps);
+    T ind2 = tau5 / (beta2 + eps);
+    T s0 = T(1) + sq(ind0), s1 = T(1) + sq(ind1), s2 = T(1) + sq(ind2);
+
+    T a0 = T(0.1) * s0, a1 = T(0.6) * s1, a2 = T(0.3) * s2;
+    T f0 = T(2) * qm2 - T(7) * qm1 + T(11) * q;
+    T f1 = -qm1 + T(5) * q + T(2) * qp1;
+    T f2 = T(2) * q + T(5) * qp1 - qp2;
+    hi = (f0 * a0 + f1 * a1 + f2 * a2) / (T(6) * (a0 + a1 + a2));
+
+    a0 = T(0.1) * s2;
+    a2 = T(0.3) * s0;
+    f0 = T(2) * qp2 - T(7) * qp1 + T(11) * q;
+    f1 = -qp1 + T(5) * q + T(2) * qm1;
+    f2 = T(2) * q + T(5) * qm1 - qm2;
+    lo = (f0 * a0 + f1 * a1 + f2 * a2) / (T(6) * (a0 + a1 + a2));
+}
+
+// fluxlimiter.jl:1-13  MINMOD(q_{i-1}, q_i, q_{i+1})
+template <class T> __device__ __forceinline__ void minmod(T qm1, T q, T qp1, T &lo, T &hi)
+{
+    T sp = qp1 - q, sm = q - qm1;
+    T dU = T(0.25) * (jsign(sp) + jsign(sm)) * jmin(dabs(sp), dabs(sm));
+    lo = q - dU;
+    hi = q + dU;
+}
+
+// fluxlimiter.jl:74-180  PPM (AthenaPK form)
+template <class T>
+__device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T &lo, T &hi)
+{
+    const T C2 = T(1.25);
+    T qa = q_i - q_im1;
+    T qb = q_ip1 - q_i;
+    T dd_im1 = T(0.5) * qa + T(0.5) * (q_im1 - q_im2);
+    T dd = T(0.5) * qb + T(0.5) * qa;
+    T dd_ip1 = T(0.5) * (q_ip2 - q_ip1) + T(0.5) * qb;
+    T dph = T(0.5) * (q_im1 + q_i) + (dd_im1 - dd) / T(6);
+    T dph_ip1 = T(0.5) * (q_i + q_ip1) + (dd - dd_ip1) / T(6);
+    T d2qc_im1 = q_im2 + q_i - T(2) * q_im1;
+    T d2qc = q_im1 + q_ip1 - T(2) * q_i;
+    T d2qc_ip1 = q_i + q_ip2 - T(2) * q_ip1;
+
+    T qa_tmp = dph - q_im1;
+    T qb_tmp = q_i - dph;
+    qa = T(3) * (q_im1 + q_i - T(2) * dph);
+    qb = d2qc_im1;
+    T qc = d2qc;
+    T qd = T(0);
+    if (jsign(qa) == jsign(qb) && jsign(qa) == jsign(qc))
+        qd = jsign(qa) * jmin(C2 * dabs(qb), jmin(C2 * dabs(qc), dabs(qa)));
+    T dph_tmp = T(0.5) * (q_im1 + q_i) - qd / T(6);
+    if (qa_tmp * qb_tmp < T(0)) dph = dph_tmp;
+
+    qa_tmp = dph_ip1 - q_i;
+    qb_tmp = q_ip1 - dph_ip1;
+    qa = T(3) * (q_i + q_ip1 - T(2) * dph_ip1);
+    qb = d2qc;
+    qc = d2qc_ip1;
+    qd = T(0);
+    if (jsign(qa) == jsign(qb) && jsign(qa) == jsign(qc))
+        qd = jsign(qa) * jmin(C2 * dabs(qb), jmin(C2 * dabs(qc), dabs(qa)));
+    T dphip1_tmp = T(0.5) * (q_i + q_ip1) - qd / T(6);
+    if (qa_tmp * qb_tmp < T(0)) dph_ip1 = dphip1_tmp;
+
+    T d2qf = T(6) * (dph + dph_ip1 - T(2) * q_i);
+    T qr_i = dph, ql_ip1 = dph_ip1;
+    T dqf_minus = q_i - qr_i;
+    T dqf_plus = ql_ip1 - q_i;
+    qa_tmp = dqf_minus * dqf_plus;
+    qb_tmp = (q_ip1 - q_i) * (q_i - q_im1);
+    qa = d2qc_im1;
+    qb = d2qc;
+    qc = d2qc_ip1;
+    qd = d2qf;
+    T qe = T(0);
+    if (jsign(qa) == jsign(qb) && jsign(qa) == jsign(qc) && jsign(qa) == jsign(qd))
+        qe = jsign(qd) * jmin(jmin(C2 * dabs(qa), C2 * dabs(qb)), jmin(C2 * dabs(qc), dabs(qd)));
+    qa = jmax(dabs(q_im1), dabs(q_im2));
+    qb = jmax(jmax(dabs(q_i), dabs(q_ip1)), dabs(q_ip2));
+    T rho = T(0);
+    if (dabs(qd) > T(1.0e-12) * jmax(qa, qb)) rho = qe / qd;
+    T tmp_m = q_i - rho * dqf_minus;
+    T tmp_p = q_i + rho * dqf_plus;
+    T tmp2_m = q_i - T(2) * dqf_plus;
+    T tmp2_p = q_i + T(2) * dqf_minus;
+    if (qa_tmp <= T(0) || qb_tmp <= T(0)) {
+        if (rho <= T(1) - T(1.0e-12)) {
+            qr_i = tmp_m;
+            ql_ip1 = tmp_p;
+        }
+    } else {
+        if (dabs(dqf_minus) >= T(2) * dabs(dqf_plus)) qr_i = tmp2_m;
+        if (dabs(dqf_plus) >= T(2) * dabs(dqf_minus)) ql_ip1 = tmp2_p;
+    }
+    lo = qr_i;
+    hi = ql_ip1;
+}
+
+template <int REC, class T>
+__device__ __forceinline__ void recon(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
+{
+    if constexpr (REC == REC_WENOZ)
+        wenoz(qm2, qm1, q, qp1, qp2, lo, hi);
+    else if constexpr (REC == REC_PPM)
+        ppm(qm2, qm1, q, qp1, qp2, lo, hi);
+    else
+        minmod(qm1, q, qp1, lo, hi);
+}
+
+// ---------------------------------------------------------------- conversions
+// 3dim/structs.jl:84-142 (NV = 5) and 2dim/structs.jl:79-119 (NV = 4):
+// lor = sqrt(sum u_i^2 + 1), w = gamma*u + rho,
+// U = (lor rho, (gamma-1)u - lor^2 w, u_i lor w), F^d as function_PtoF{x,y,z}.
+template <int NV, class T> __device__ __forceinline__ T lorentz(const T *P)
+{
+    if constexpr (NV == 5)
+        return dsqrt(P[2] * P[2] + P[3] * P[3] + P[4] * P[4] + T(1));
+    else
+        return dsqrt(P[2] * P[2] + P[3] * P[3] + T(1));
+}
+
+template <int NV, class T> __device__ __forceinline__ void p2u(const T *P, T *U, T g, T gam)
+{
+    T w = g * P[1] + P[0];
+    U[0] = gam * P[0];
+    U[1] = (g - T(1)) * P[1] - gam * gam * w;
+    U[2] = P[2] * gam * w;
+    U[3] = P[3] * gam * w;
+    if constexpr (NV == 5) U[4] = P[4] * gam * w;
+}
+
+template <int NV, int DIM, class T>
+__device__ __forceinline__ void p2f(const T *P, T *F, T g, T gam)
+{
+    T w = g * P[1] + P[0];
+    T pr = (g - T(1)) * P[1];
+    if constexpr (DIM == 0) {
+        F[0] = P[0] * P[2];
+        F[1] = -w * P[2] * gam;
+        F[2] = P[2] * P[2] * w + pr;
+        F[3] = P[2] * P[3] * w;
+        if constexpr (NV == 5) F[4] = P[2] * P[4] * w;
+    } else if constexpr (DIM == 1) {
+        F[0] = P[0] * P[3];
+        F[1] = -w * P[3] * gam;
+        F[2] = P[2] * P[3] * w;
+        F[3] = P[3] * P[3] * w + pr;
+        if constexpr (NV == 5) F[4] = P[4] * P[3] * w;
+    } else {
+        F[0] = P[0] * P[4];
+        F[1] = -w * P[4] * gam;
+        F[2] = P[2] * P[4] * w;
+        F[3] = P[3] * P[4] * w;
+        F[4] = w * (P[4] * P[4]) + pr;
+    }
+}
+
+// eos.jl:15-17
+template <class T> __device__ __forceinline__ T sound_speed(T rho, T u, T g)
+{
+    return dsqrt((g * (g - T(1)) * u) / (rho + g * u));
+}
+
+// 3dim/algos.jl:146-157: sigma = c^2/(lor^2 (1-c^2)),
+// lam+- = (v +- sqrt(sigma (1 - v^2 + sigma)))/(1 + sigma)
+template <class T>
+__device__ __forceinline__ void lambdas(T v, T cs, T lor2, T &lp, T &lm)
+{
+    T sig = cs * cs / (lor2 * (T(1) - cs * cs));
+    T rt = dsqrt(sig * (T(1) - v * v + sig));
+    lp = (v + rt) / (T(1) + sig);
+    lm = (v - rt) / (T(1) + sig);
+}
+
+// HLL (3dim/algos.jl:158-174); HLLC with HLL fallback (Mignone & Bodo 2005,
+// no reference counterpart) written in Verona's variables (U[1] = -E, F[1] = -m_d).
+template <int NV, int RIEM, class T>
+__device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T *FL, const T *FR,
+                                              T vL, T vR, T pL, T pR, T Cmax, T Cmin, int nd, T *F)
+{
+    if (Cmax < T(0)) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) F[v] = FR[v];
+        return;
+    }
+    if (Cmin < T(0)) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) F[v] = FL[v];
+        return;
+    }
+    if constexpr (RIEM == RIEM_HLLC) {
+        T SL = -Cmin, SR = Cmax;
+        T inv = T(1) / (Cmax + Cmin);
+        T E_h = -(Cmax * UR[1] + Cmin * UL[1] + FL[1] - FR[1]) * inv;
+        T m_h = (Cmax * UR[nd] + Cmin * UL[nd] + FL[nd] - FR[nd]) * inv;
+        T FE_h = -(FR[1] * Cmin + FL[1] * Cmax - Cmax * Cmin * (UR[1] - UL[1])) * inv;
+        T Fm_h = (FR[nd] * Cmin + FL[nd] * Cmax - Cmax * Cmin * (UR[nd] - UL[nd])) * inv;
+        T b = E_h + Fm_h;
+        T disc = b * b - T(4) * FE_h * m_h;
+        bool ok = disc >= T(0);
+        T lam = T(0), ps = T(0);
+        if (ok) {
+            T qq = b + dsqrt(disc);
+            lam = T(2) * m_h / qq;
+            ps = Fm_h - lam * FE_h;
+            ok = __builtin_isfinite(lam) && lam > SL && lam < SR && ps > T(0);
+        }
+        if (ok) {
+            bool left = lam >= T(0);
+            T S = left ? SL : SR;
+            T vK = left ? vL : vR;
+            T pK = left ? pL : pR;
+            T fac = T(1) / (S - lam);
+            T sv = S - vK;
+            T Us[NV];
+            T UK[NV], FK[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                UK[v] = left ? UL[v] : UR[v];
+                FK[v] = left ? FL[v] : FR[v];
+            }
+            Us[0] = UK[0] * sv * fac;
+            T Es = (-UK[1] * sv + ps * lam - pK * vK) * fac;
+            Us[1] = -Es;
+#pragma unroll
+            for (int v = 2; v < NV; v++)
+                Us[v] = (v == nd) ? (UK[v] * sv + ps - pK) * fac : UK[v] * sv * fac;
+            if (Us[0] > T(0) && Es > T(0)) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) F[v] = FK[v] + S * (Us[v] - UK[v]);
+                return;
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        F[v] = (FR[v] * Cmin + FL[v] * Cmax - Cmax * Cmin * (UR[v] - UL[v])) / (Cmax + Cmin);
+}
+
+// Face flux along DIM from floored face states PL (left) and PR (right):
+// 3dim/algos.jl:117-174 (NV = 5), 2dim/algos.jl:115-166 (NV = 4).
+template <int NV, int DIM, int RIEM, class T>
+__device__ __forceinline__ void face_flux(const T *PL, const T *PR, T g, T *F)
+{
+    T UL[NV], UR[NV], FL[NV], FR[NV];
+    T lorL = lorentz<NV>(PL), lorR = lorentz<NV>(PR);
+    p2u<NV>(PR, UR, g, lorR);
+    p2u<NV>(PL, UL, g, lorL);
+    p2f<NV, DIM>(PR, FR, g, lorR);
+    p2f<NV, DIM>(PL, FL, g, lorL);
+    T vL = PL[2 + DIM] / lorL;
+    T vR = PR[2 + DIM] / lorR;
+    T CL = sound_speed(PL[0], PL[1], g);
+    T CR = sound_speed(PR[0], PR[1], g);
+    T lpL, lmL, lpR, lmR;
+    lambdas(vL, CL, lorL * lorL, lpL, lmL);
+    lambdas(vR, CR, lorR * lorR, lpR, lmR);
+    T Cmax = jmax(lpL, lpR);
+    T Cmin = -jmin(lmL, lmR);
+    riemann_solve<NV, RIEM>(UL, UR, FL, FR, vL, vR, (g - T(1)) * PL[1], (g - T(1)) * PR[1], Cmax,
+                            Cmin, 2 + DIM, F);
+}
+
+// Cell-centred signal-speed rate sum_d max(|lam+_d|, |lam-_d|)/dx_d used by
+// the CFL reduction (wave speeds as in 3dim/algos.jl:130-157).
+template <int NV>
+__device__ __forceinline__ double cell_rate(const double *P, double g, const double *inv_dx)
+{
+    double lor = lorentz<NV>(P);
+    double cs = sound_speed(P[0], P[1], g);
+    double r = 0.0;
+#pragma unroll
+    for (int d = 0; d < NV - 2; d++) {
+        double lp, lm;
+        lambdas(P[2 + d] / lor, cs, lor * lor, lp, lm);
+        r = r + jmax(dabs(lp), dabs(lm)) * inv_dx[d];
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- con2prim
+// 3dim/structs.jl:273-343  1DW Newton (Noble et al. 2006) from the warm-start
+// guess Pg.  Returns true and fills P (recovery + floors + Gamma cap,
+// :598-609) on convergence; false means the cascade must continue.
+__device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, double g, int n_iter,
+                                        double tol, double *P, int &iters)
+{
+    const double D = U[0], Qt = U[1], Qx = U[2], Qy = U[3], Qz = U[4];
+    const double c = (g - 1.0) / g;
+    double S2 = Qx * Qx + Qy * Qy + Qz * Qz;
+    double gm = dsqrt(Pg[2] * Pg[2] + Pg[3] * Pg[3] + Pg[4] * Pg[4] + 1.0);
+    double W = (Pg[0] + g * Pg[1]) * (gm * gm);
+    const double W_max = 1e30;
+    double W_min = dsqrt(S2) * (1.0 + 1e-15);
+    for (int t10 = 0; t10 < 1100 && S2 / (W * W) >= 1.0 && W < W_max; t10++) W *= 10.0; /* unbounded in the reference; W = 0 would never exit */
+    bool conv = false;
+    for (int it = 0; it < n_iter; it++) {
+        iters++;
+        if (W < W_min) W = W_min;
+        double v2 = S2 / (W * W);
+        if (v2 < 0.0)
+            v2 = 0.0;
+        else if (v2 > 1.0 - 1e-15)
+            v2 = 1.0 - 1e-15;
+        double sv = dsqrt(1.0 - v2);
+        double fun = Qt + W - c * (W * (1.0 - v2) - D * sv);
+        double W3 = W * W * W;
+        double jac = 1.0 - c * (1.0 - v2) - c * (W * (2.0 * S2 / W3) - D * (S2 / (W3 * sv)));
+        double dW = fun / jac;
+        double Wp = W - dW;
+        W = (Wp < W_min) ? 0.5 * (W + W_min) : Wp;
+        if (dW < 0.0) dW = -dW;
+        if (dW * dW < tol * tol) {
+            conv = true;
+            break;
+        }
+    }
+    if (!conv) return false;
+    double v2 = S2 / (W * W);
+    double gam = 1.0 / dsqrt(1.0 - v2);
+    gam = jmin(gam, 50.0);
+    P[0] = jmax(1e-8, D / gam);
+    P[1] = jmax(1e-8, (W / (gam * gam) - D / gam) / g);
+    P[2] = gam * Qx / W;
+    P[3] = gam * Qy / W;
+    P[4] = gam * Qz / W;
+    return true;
+}
+
+// 3dim/structs.jl:346-628: the rest of the cascade after 1DW failed --
+// 2D Newton (W, v^2) with backtracking, 5D Newton on primitives, bisection on
+// W, recovery, or the NaN marker.  P holds the guess on entry.  Returns the
+// method that finished the cell (2, 3, 4) or 0 for failure.
+__device__ __noinline__ int c2p_cascade_tail(const double *U, double *P, double g, int n_iter,
+                                             double tol, int &iters)
+{
+    const double D = U[0], Qt = U[1], Qx = U[2], Qy = U[3], Qz = U[4];
+    const double c = (g - 1.0) / g;
+    double S2 = Qx * Qx + Qy * Qy + Qz * Qz;
+    double W, v2;
+    bool conv2 = false, conv5 = false, convb = false;
+    {
+        double gm = dsqrt(P[2] * P[2] + P[3] * P[3] + P[4] * P[4] + 1.0);
+        W = (P[0] + g * P[1]) * (gm * gm);
+        const double W_max = 1e30;
+        for (int t10 = 0; t10 < 1100 && S2 / (W * W) >= 1.0 && W < W_max; t10++) W *= 10.0; /* unbounded in the reference; W = 0 would never exit */
+        v2 = jmin(S2 / (W * W), 1.0 - 1e-15);
+        for (int it = 0; it < n_iter; it++) {
+            iters++;
+            double W_old = W, v2_old = v2;
+            double f0 = v2 - S2 / (W * W);
+            double f1 = Qt + W - c * (W * (1.0 - v2) - D * dsqrt(1.0 - v2));
+            double J0 = 2.0 * S2 / (W * W * W);
+            double J1 = 1.0 - c * (1.0 - v2);
+            double J2 = 1.0;
+            double J3 = c * (W - D / (2.0 * dsqrt(1.0 - v2)));
+            // 3dim/structs.jl:181-214  LU_dec_2D!
+            J1 /= J0;
+            J3 -= J1 * J2;
+            double x1 = f1;
+            double x0 = f0;
+            x1 -= J1 * x0;
+            x1 /= J3;
+            x0 -= J2 * x1;
+            x0 /= J0;
+            double dW = x0, dv2 = x1, alpha = 1.0;
+            bool accepted = false;
+            for (int b = 0; b < 10; b++) {
+                double sqv = dsqrt(W - alpha * dW);
+                double Wc = sqv * sqv;
+                double v2c = v2 - alpha * dv2;
+                if (Wc <= 0.0 || Wc > W_max) Wc = W_old;
+                if (v2c < 0.0)
+                    v2c = 0.0;
+                else if (v2c >= 1.0)
+                    v2c = 1.0 - 1e-15;
+                double r1 = v2c - S2 / (Wc * Wc);
+                double r2 = Qt + Wc - c * (Wc * (1.0 - v2c) - D * dsqrt(1.0 - v2c));
+                double errc = r1 * r1 + r2 * r2;
+                double r1o = v2 - S2 / (W * W);
+                double r2o = Qt + W - c * (W * (1.0 - v2) - D * dsqrt(1.0 - v2));
+                double erro = r1o * r1o + r2o * r2o;
+                if (errc <= erro) {
+                    W = Wc;
+                    v2 = v2c;
+                    accepted = true;
+                    break;
+                }
+                alpha *= 0.5;
+            }
+            if (!accepted) {
+                W = W_old;
+                v2 = v2_old;
+            }
+            double relW = dsqrt((W - W_old) * (W - W_old)) / jmax(dabs(W_old), tol * tol);
+            double relv = dsqrt((v2 - v2_old) * (v2 - v2_old)) / jmax(dabs(v2_old), tol * tol);
+            if (relW + relv < tol) {
+                conv2 = true;
+                break;
+            }
+        }
+    }
+    if (!conv2) {
+        for (int it = 0; it < n_iter; it++) {
+            iters++;
+            double gam = dsqrt(P[2] * P[2] + P[3] * P[3] + P[4] * P[4] + 1.0);
+            double w = g * P[1] + P[0];
+            double f[5], J[25], o[5];
+            f[0] = gam * P[0] - U[0];
+            f[1] = (g - 1.0) * P[1] - gam * gam * w - U[1];
+            f[2] = P[2] * gam * w - U[2];
+            f[3] = P[3] * gam * w - U[3];
+            f[4] = P[4] * gam * w - U[4];
+            J[0] = gam;
+            J[5] = 0.0;
+            J[10] = P[2] * P[0] / gam;
+            J[15] = P[3] * P[0] / gam;
+            J[20] = P[4] * P[0] / gam;
+            J[1] = -(gam * gam);
+            J[6] = g * (-(gam * gam)) + g - 1.0;
+            J[11] = -2.0 * P[2] * w;
+            J[16] = -2.0 * P[3] * w;
+            J[21] = -2.0 * P[4] * w;
+            J[2] = P[2] * gam;
+            J[7] = g * P[2] * gam;
+            J[12] = P[2] * P[2] * w / gam + w * gam;
+            J[17] = P[2] * P[3] * w / gam;
+            J[22] = P[2] * P[4] * w / gam;
+            J[3] = P[3] * gam;
+            J[8] = g * P[3] * gam;
+            J[13] = P[2] * P[3] * w / gam;
+            J[18] = P[3] * P[3] * w / gam + w * gam;
+            J[23] = P[3] * P[4] * w / gam;
+            J[4] = P[4] * gam;
+            J[9] = g * P[4] * gam;
+            J[14] = P[2] * P[4] * w / gam;
+            J[19] = P[3] * P[4] * w / gam;
+            J[24] = P[4] * P[4] * w / gam + w * gam;
+            // 3dim/structs.jl:144-179  LU_dec_5D! (no pivoting)
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+#pragma unroll
+                for (int i = k + 1; i < 5; i++) {
+                    J[k * 5 + i] /= J[k * 5 + k];
+#pragma unroll
+                    for (int j = k + 1; j < 5; j++) J[j * 5 + i] -= J[k * 5 + i] * J[j * 5 + k];
+                }
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                o[i] = f[i];
+#pragma unroll
+                for (int j = 0; j < i; j++) o[i] -= J[j * 5 + i] * o[j];
+            }
+#pragma unroll
+            for (int i = 4; i >= 0; i--) {
+#pragma unroll
+                for (int j = i + 1; j < 5; j++) o[i] -= J[j * 5 + i] * o[j];
+                o[i] /= J[i * 5 + i];
+            }
+            if (o[0] * o[0] + o[1] * o[1] + o[2] * o[2] + o[3] * o[3] + o[4] * o[4] < tol * tol) {
+                if (__builtin_isfinite(P[0]) && __builtin_isfinite(P[1]) &&
+                    __builtin_isfinite(P[2]) && __builtin_isfinite(P[3]) &&
+                    __builtin_isfinite(P[4])) {
+                    conv5 = true;
+                    P[0] = jmax(1e-8, P[0]);
+                    P[1] = jmax(1e-8, P[1]);
+                    double gm = dsqrt(P[2] * P[2] + P[3] * P[3] + P[4] * P[4] + 1.0);
+                    P[2] = P[2] / gm;
+                    P[3] = P[3] / gm;
+                    P[4] = P[4] / gm;
+                    gm = jmin(gm, 50.0);
+                    P[2] = P[2] * gm;
+                    P[3] = P[3] * gm;
+                    P[4] = P[4] * gm;
+                }
+                break;
+            }
+#pragma unroll
+            for (int v = 0; v < 5; v++) P[v] = P[v] - o[v];
+        }
+    }
+    if (!conv2 && !conv5) {
+        auto res = [&](double Wv) {
+            return Qt + Wv - c * (Wv * (1.0 - S2 / (Wv * Wv)) - D * dsqrt(jmax(0.0, 1.0 - S2 / (Wv * Wv))));
+        };
+        double W_min = jmax(1e-15, dsqrt(S2));
+        double W_max = W_min * 10.0;
+        double fmin = res(W_min), fmax = res(W_max);
+        int count = 0;
+        while (fmin * fmax > 0.0 && count < 100) {
+            W_max *= 10.0;
+            fmax = res(W_max);
+            count++;
+        }
+        if (fmin * fmax <= 0.0) {
+            const double thr = 1e-6 * 1e-6;
+            for (int it = 0; it < n_iter * 10; it++) {
+                iters++;
+                double W_mid = 0.5 * (W_min + W_max);
+                double fmid = res(W_mid);
+                if (fmid * fmid < thr) {
+                    convb = true;
+                    W = W_mid;
+                    break;
+                }
+                if (fmin * fmid < 0.0) {
+                    W_max = W_mid;
+                    fmax = fmid;
+                } else {
+                    W_min = W_mid;
+                    fmin = fmid;
+                }
+                W = W_mid;
+            }
+        }
+    }
+    if (conv2 || convb) {
+        v2 = S2 / (W * W);
+        double gam = 1.0 / dsqrt(1.0 - v2);
+        gam = jmin(gam, 50.0);
+        P[0] = jmax(1e-8, D / gam);
+        P[1] = jmax(1e-8, (W / (gam * gam) - D / gam) / g);
+        P[2] = gam * Qx / W;
+        P[3] = gam * Qy / W;
+        P[4] = gam * Qz / W;
+    }
+    if (!conv2 && !conv5 && !convb) {
+        const double nan = __builtin_nan("");
+#pragma unroll
+        for (int v = 0; v < 5; v++) P[v] = nan;
+        return 0;
+    }
+    return conv2 ? 2 : (conv5 ? 3 : 4);
+}
+
+// 2dim/structs.jl:249-330  4x4 Newton on (rho, u, ux, uy), warm start in P,
+// LU with row-scaled partial pivoting and explicit fma (:121-208).  Returns
+// false if a row of the Jacobian is identically zero (the reference raises).
+__device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int n_iter, double tol,
+                                       int &iters)
+{
+    for (int it = 0; it < n_iter; it++) {
+        iters++;
+        double gam = dsqrt(P[2] * P[2] + P[3] * P[3] + 1.0);
+        double w = g * P[1] + P[0];
+        double f[4], A[16], o[4], rn[4];
+        f[0] = gam * P[0] - U[0];
+        f[1] = (g - 1.0) * P[1] - gam * gam * w - U[1];
+        f[2] = P[2] * gam * w - U[2];
+        f[3] = P[3] * gam * w - U[3];
+        A[0] = gam;
+        A[4] = 0.0;
+        A[8] = P[0] * P[2] / gam;
+        A[12] = P[0] * P[3] / gam;
+        A[1] = -(gam * gam);
+        A[5] = (g - 1.0) - gam * gam * g;
+        A[9] = -2.0 * P[2] * w;
+        A[13] = -2.0 * P[3] * w;
+        A[2] = gam * P[2];
+        A[6] = gam * P[2] * g;
+        A[10] = (2.0 * (P[2] * P[2]) + P[3] * P[3] + 1.0) / gam * w;
+        A[14] = P[2] * P[3] / gam * w;
+        A[3] = gam * P[3];
+        A[7] = gam * P[3] * g;
+        A[11] = P[2] * P[3] / gam * w;
+        A[15] = (2.0 * (P[3] * P[3]) + P[2] * P[2] + 1.0) / gam * w;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double am = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                double t = dabs(A[j * 4 + i]);
+                if (t > am) am = t;
+            }
+            if (am == 0.0) return false;
+            rn[i] = 1.0 / am;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int piv = k;
+            double pv = dabs(A[k * 4 + k]) * rn[k];
+#pragma unroll
+            for (int i = k + 1; i < 4; i++) {
+                double t = dabs(A[k * 4 + i]) * rn[i];
+                if (t > pv) {
+                    pv = t;
+                    piv = i;
+                }
+            }
+            // Row swap written as selects so the arrays stay in registers.
+#pragma unroll
+            for (int i = k + 1; i < 4; i++) {
+                bool s = (piv == i);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    double a = A[j * 4 + k], b = A[j * 4 + i];
+                    A[j * 4 + k] = s ? b : a;
+                    A[j * 4 + i] = s ? a : b;
+                }
+                double a = f[k], b = f[i];
+                f[k] = s ? b : a;
+                f[i] = s ? a : b;
+                a = rn[k];
+                b = rn[i];
+                rn[k] = s ? b : a;
+                rn[i] = s ? a : b;
+            }
+#pragma unroll
+            for (int i = k + 1; i < 4; i++) {
+                A[k * 4 + i] /= A[k * 4 + k];
+#pragma unroll
+                for (int j = k + 1; j < 4; j++)
+                    A[j * 4 + i] = __builtin_fma(-A[k * 4 + i], A[j * 4 + k], A[j * 4 + i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            o[i] = f[i];
+#pragma unroll
+            for (int j = 0; j < i; j++) o[i] = __builtin_fma(-A[j * 4 + i], o[j], o[i]);
+        }
+#pragma unroll
+        for (int i = 3; i >= 0; i--) {
+#pragma unroll
+            for (int j = i + 1; j < 4; j++) o[i] = __builtin_fma(-A[j * 4 + i], o[j], o[i]);
+            o[i] /= A[i * 4 + i];
+        }
+        if (dsqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2] + o[3] * o[3]) < tol) break;
+#pragma unroll
+        for (int v = 0; v < 4; v++) P[v] = P[v] - o[v];
+    }
+    return true;
+}
+
+}  // namespace vrn

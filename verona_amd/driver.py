@@ -1,0 +1,94 @@
+"""Run loop shared by the 2D and 3D drivers (Verona2D/3D.HARM_HLL).
+
+Follows src/3dim/algos.jl:203-362 (2D: src/2dim/algos.jl:192-324): halo
+exchange and PtoU at the start, dump 0, then ``while t < Tmax``: boundary
+callback, one fused time step on the GPU, ``t += dt``, and a dump whenever
+``t`` passes the next multiple of ``drops``.  Returns the dump count.
+"""
+import os
+import time
+
+import numpy as np
+
+from .bc import Boundary, Cart
+
+
+def _kinds(comm, fun_bound, ndim):
+    periodic = getattr(comm, "periodic", (False,) * ndim) if comm is not None else (False,) * ndim
+    periodic = tuple(periodic) + (False,) * (3 - len(periodic))
+    if isinstance(fun_bound, Boundary):
+        return fun_bound.kinds(periodic), None
+    kinds = ["periodic" if periodic[k // 2] else "none" for k in range(6)]
+    return kinds, (fun_bound if callable(fun_bound) else None)
+
+
+def save_dump(path, Phost, t, grid):
+    """Interior primitives as in SaveHDF5Parallel (src/3dim/saving.jl:57-146):
+    dataset ``data`` (Julia shape (nvar, Nx, Ny[, Nz])), ``T`` and ``grid``;
+    raises on NaN like src/3dim/saving.jl:70-72."""
+    sl = (slice(None),) + (slice(3, -3),) * (Phost.ndim - 1)
+    data = np.ascontiguousarray(Phost[sl])
+    if np.isnan(data).any():
+        raise RuntimeError("Nan in matrix")
+    np.savez(path, data=data, T=np.float64(t), grid=np.asarray(grid, dtype=np.float64))
+
+
+def harm_hll(dev, ndim, comm, P_host_view, dt, grid, Tmax, eos, drops, floor, out_dir, verbose,
+             n_iter, tol, fun_bound, *, rec="wenoz", riemann="hll", integrator="rk2", cfl=0.0,
+             precision="fp64", ref_mode=True, max_steps=None):
+    kinds, callback = _kinds(comm if comm is not None else Cart(), fun_bound, ndim)
+    nozzle = getattr(fun_bound, "nozzle", None)
+    phi_seq = None
+    if nozzle is not None:
+        dev.set_nozzle(nozzle["radius"], grid[0], grid[1], nozzle["rho"], nozzle["u"],
+                       nozzle["umag"], nozzle["eps"])
+        phi_seq = nozzle.get("phi_seq")
+    dx, dy = grid[0], grid[1]
+    dz = grid[2] if ndim == 3 else 1.0
+    # start: halo exchange (periodic axes) [+ boundaries in the extended form], PtoU
+    dev.fill_ghosts([k if k == "periodic" else "none" for k in kinds])
+    if not ref_mode:
+        dev.fill_ghosts(kinds)
+    dev.prim_to_cons()
+    dev.set_step(ref_mode=ref_mode, rec=rec, riemann=riemann, integrator=integrator,
+                 precision=precision, n_iter=n_iter, tol=tol, floor=floor, dt=dt, cfl=cfl,
+                 dx=dx, dy=dy, dz=dz,
+                 bc=[("none" if (callback is not None and k != "periodic") else k) for k in kinds])
+    t = 0.0
+    i = 0
+    coords = (0,) * ndim
+    dims = (1,) * ndim
+    if out_dir is not None:
+        os.makedirs(out_dir, exist_ok=True)
+        save_dump(os.path.join(out_dir, f"dump{i}.npz"), dev.download("P"), t, grid)
+    thres = drops
+    t0 = time.time()
+    steps = 0
+    ncell = int(np.prod([s for s in dev.shape[1:]]))
+    while t < Tmax:
+        if max_steps is not None and steps >= max_steps:
+            break
+        if callback is not None:
+            Ph = dev.download("P")
+            if ndim == 3:
+                callback(Ph, t, coords, dims)
+            else:
+                callback(Ph, t)
+            dev.upload("P", Ph)
+        phi = phi_seq[steps] if phi_seq is not None else 0.0
+        dt_used = dev.step(phi=phi, want_dt=cfl > 0)
+        t += dt_used if cfl > 0 else dt
+        steps += 1
+        if t > thres:
+            i += 1
+            thres += drops
+            if verbose:
+                dev.sync()
+                el = time.time() - t0
+                print(f"{t:.3g} elapsed: {el:.3g} s")
+                print(f"speed: {ncell * drops / dt / el * 1e-6:.3g} zones [10^6/s]")
+                t0 = time.time()
+            if out_dir is not None:
+                save_dump(os.path.join(out_dir, f"dump{i}.npz"), dev.download("P"), t, grid)
+    dev.sync()
+    return i
