@@ -1,0 +1,195 @@
+"""Benchmark: zone-updates/s of the 3D relativistic-jet workload on one MI355X.
+
+Workload (fits one GPU): 384^3 interior cells on [0,1]^3, gamma = 4/3, ambient
+rho = 1, p = 1e-4 at rest; jet nozzle of radius 8 cells on the z-low face
+(rho = 0.01, p = 1e-4, Lorentz factor 10 along e_z with a 1% precession angle
+drawn per step from SplitMix64 seed 42); outflow elsewhere.  WENOZ + HLLC +
+SSP-RK3, CFL 0.4, all FP64.  One step = one full SSP-RK3 time step of the
+whole grid (inputs resident in HBM); value = interior cells x steps / s.
+
+Also timed, same JSON line: the reference-semantics form (WENOZ + HLL +
+midpoint RK2, fixed dt) and the FP32 reconstruction/Riemann form, plus the
+CPU oracle on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 384]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+GAMMA = 4.0 / 3.0
+# compulsory HBM bytes per interior zone-update (FP64 P and U, 5 variables):
+# SSP-RK3 = 14 array passes, midpoint RK2 = 8 array passes
+BYTES_PER_ZU = {"rk3": 14 * 40, "rk2": 8 * 40}
+
+
+def jet_state(n):
+    """Ambient medium at rest, Verona layout (5, n+6, n+6, n+6)."""
+    from verona_amd import host_array
+    P = host_array(3, n, n, n)
+    P[0] = 1.0
+    P[1] = 1e-4 / (GAMMA - 1.0)
+    return P
+
+
+def nozzle(n):
+    dx = 1.0 / n
+    return dict(radius=8 * dx, rho=0.01, u=1e-4 / (GAMMA - 1.0), umag=math.sqrt(99.0), eps=0.01)
+
+
+def phi_stream(count, seed=42):
+    from verona_amd.bc import uniform_stream
+    it = uniform_stream(seed)
+    return [2.0 * math.pi * next(it) for _ in range(count)]
+
+
+BC_JET = ["outflow", "outflow", "outflow", "outflow", "nozzle", "outflow"]
+
+
+def run_gpu(n, steps, warmup, *, ref, precision="fp64", timing=True):
+    from verona_amd import DeviceGrid
+    dx = 1.0 / n
+    nz = nozzle(n)
+    phis = phi_stream(steps + warmup)
+    dev = DeviceGrid(3, n, n, n, GAMMA)
+    try:
+        dev.upload("P", jet_state(n))
+        dev.set_nozzle(nz["radius"], dx, dx, nz["rho"], nz["u"], nz["umag"], nz["eps"])
+        if ref:
+            dt = 0.4 / (3.0 / dx)
+            dev.set_step(ref_mode=True, rec="wenoz", riemann="hll", integrator="rk2",
+                         precision=precision, n_iter=50, tol=1e-10, floor=1e-7, dt=dt,
+                         dx=dx, dy=dx, dz=dx, bc=BC_JET)
+        else:
+            dev.fill_ghosts(BC_JET)
+            dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3",
+                         precision=precision, n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4,
+                         dx=dx, dy=dx, dz=dx, bc=BC_JET)
+        dev.prim_to_cons()
+        for s in range(warmup):
+            dev.step(phi=phis[s])
+        dev.sync()
+        if timing:
+            dev.timing(True)
+        t0 = time.perf_counter()
+        for s in range(steps):
+            dev.step(phi=phis[warmup + s])
+        dev.sync()
+        el = time.perf_counter() - t0
+        rep = dev.timing_report() if timing else {}
+        if timing:
+            dev.timing(False)
+        nbad, nresc = dev.counters()
+    finally:
+        dev.close()
+    return el, rep, nbad, nresc
+
+
+def cpu_baseline(budget_s=15.0, n=128):
+    """CPU oracle (plain-C restatement of Verona's algorithm, OpenMP) on a
+    bounded sample of the same workload: an n^3 grid, SSP-RK3 steps until
+    about budget_s seconds of wall time have elapsed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    P = jet_state(n)
+    Po = np.ascontiguousarray(P.T)
+    dx = 1.0 / n
+    nz = nozzle(n)
+    mode = O.make_mode(ref=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50,
+                       tol=1e-10, gamma=GAMMA, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx)
+    phis = phi_stream(4096)
+    bc, keep = O.make_bc3([2, 2, 2, 2, 3, 2], dx=dx, dy=dx,
+                          nozzle=(nz["radius"], nz["rho"], nz["u"], nz["umag"], nz["eps"], 0.0),
+                          phi_seq=phis)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        O.run3(Po, mode, bc, 2)
+        steps += 2
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 400:
+            break
+    return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": cores, "kind": "port",
+            "sample": f"CPU oracle (C restatement of Verona's algorithm, OpenMP) on the same jet "
+                      f"workload at {n}^3, {steps} SSP-RK3 WENOZ+HLLC steps, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=384)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the grid would shard).", file=sys.stderr)
+        sys.exit(2)
+    n, K, W = args.n, args.steps, args.warmup
+    cells = n ** 3
+    el, rep, nbad, nresc = run_gpu(n, K, W, ref=False)
+    value = cells * K / el
+    ms_step = el / K * 1e3
+    # dominant kernel by total time in the timed region (HIP events on the library stream)
+    tot = {k: c * m for k, (c, m) in rep.items()}
+    dom = max(tot, key=tot.get)
+    cnt, mean_ms = rep[dom]
+    # per-launch compulsory bytes of the dominant kernel's I/O contract (per interior cell)
+    per_cell = {"zsweep_c2p": 5 * 40, "xsweep": 3 * 40, "ysweep": 3 * 40}.get(dom, 4 * 40)
+    ach = per_cell * cells / (mean_ms * 1e-3) / 1e9
+    step_gbs = BYTES_PER_ZU["rk3"] * value / 1e9
+    out = {
+        "metric": "zone-updates/s (3D WENOZ+HLLC, 1 GPU); con2prim cells/s learned vs Newton",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D relativistic jet {n}^3, WENOZ+HLLC+SSP-RK3, CFL 0.4, FP64",
+                   "grid": [n, n, n], "gamma": GAMMA, "nozzle_radius_cells": 8,
+                   "lorentz": 10.0, "bc": "nozzle z-low, outflow elsewhere"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_cell_per_launch": per_cell, "mean_launch_ms": mean_ms,
+                     "launches": cnt},
+        "roofline_step": {"bytes_per_zone_update": BYTES_PER_ZU["rk3"], "achieved": step_gbs,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_ms": {k: {"launches": c, "mean_ms": m} for k, (c, m) in rep.items()},
+        "n_unrecovered": nbad,
+        "n_rescued": nresc,
+    }
+    if not args.no_extra:
+        el2, rep2, nb2, _ = run_gpu(n, max(K // 2, 4), 2, ref=True, timing=False)
+        out["ref_form"] = {"what": "Verona semantics: WENOZ+HLL, midpoint RK2, fixed dt",
+                           "value": cells * max(K // 2, 4) / el2, "unit": "zone-updates/s",
+                           "n_unrecovered": nb2}
+        el3, _, nb3, _ = run_gpu(n, max(K // 2, 4), 2, ref=False, precision="fp32", timing=False)
+        out["fp32_form"] = {"what": "reconstruction + Riemann in FP32, con2prim + accumulation FP64",
+                            "value": cells * max(K // 2, 4) / el3, "unit": "zone-updates/s",
+                            "n_unrecovered": nb3}
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

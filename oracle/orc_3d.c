@@ -578,6 +578,30 @@ double orc3_max_rate(const double *P, int NX, int NY, int NZ, double g, double d
     return mx;
 }
 
+static double cell_rate3(const double *p, double g, const double *inv)
+{
+    double lor = sqrt(p[2] * p[2] + p[3] * p[3] + p[4] * p[4] + 1.0);
+    double cs = orc_sound_speed(p[0], p[1], g);
+    double r = 0.0;
+    for (int d = 0; d < 3; d++) {
+        double lp, lm;
+        orc_lambdas(p[2 + d] / lor, cs, lor * lor, &lp, &lm);
+        r = r + jl_max(fabs(lp), fabs(lm)) * inv[d];
+    }
+    return r;
+}
+
+/* Signal rate of the nozzle's inflow state at the current angle. */
+static double nozzle_rate(const orc_bc3 *bc, double g, double dx, double dy, double dz)
+{
+    double nx = bc->noz_eps * cos(bc->noz_phi), ny = bc->noz_eps * sin(bc->noz_phi), nz = 1.0;
+    double nn = sqrt(nx * nx + ny * ny + nz * nz);
+    double p[5] = {bc->noz_rho, bc->noz_u, bc->noz_umag * (nx / nn), bc->noz_umag * (ny / nn),
+                   bc->noz_umag * (nz / nn)};
+    double inv[3] = {1.0 / dx, 1.0 / dy, 1.0 / dz};
+    return cell_rate3(p, g, inv);
+}
+
 /* One stage of the fused form: Uout = a*U0 + b*acc with
  * acc = ((Ubase - dt/dx dFx) - dt/dy dFy) - dt/dz dFz  (a == 0: Uout = acc). */
 static void stage_update3(const double *U0, const double *Ub, double *Uout, double a, double b,
@@ -630,8 +654,15 @@ long orc3_run(double *P, int NX, int NY, int NZ, const orc_mode *mode, const orc
 
     for (int s = 0; s < nsteps; s++) {
         if (bcs.type[4] == ORC_BC_NOZZLE && bc->phi_seq) bcs.noz_phi = bc->phi_seq[s];
-        if (mode->cfl > 0.0)
-            dt = mode->cfl / orc3_max_rate(P, NX, NY, NZ, g, dx, dy, dz);
+        if (mode->cfl > 0.0) {
+            double r = orc3_max_rate(P, NX, NY, NZ, g, dx, dy, dz);
+            if (bcs.type[4] == ORC_BC_NOZZLE) {
+                /* the nozzle inflow state enters through the ghost faces */
+                double ex = nozzle_rate(&bcs, g, dx, dy, dz);
+                if (ex > r) r = ex;
+            }
+            dt = mode->cfl / r;
+        }
         if (mode->ref) apply_bc3(P, NX, NY, NZ, &bcs, 1);
         int nst = mode->integrator == 1 ? 3 : 2;
         for (int st = 0; st < nst; st++) {

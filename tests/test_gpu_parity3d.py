@@ -228,3 +228,44 @@ def test_fp32_form_error_is_bounded(oracle):
     errs = per_var_rel_l1(interior(Pg), interior(Po))
     progress(f"  fp32 per-var rel L1 vs fp64 oracle {['%.2e' % e for e in errs]}")
     assert max(errs) < 1e-5
+
+
+def test_jet_nozzle_cfl_ext_mode(oracle):
+    """The jet workload at small size: nozzle inflow on z-low with a per-step
+    precession angle, outflow elsewhere, HLLC + SSP-RK3 + CFL dt."""
+    import math
+
+    from verona_amd import DeviceGrid, host_array
+    from verona_amd.bc import uniform_stream
+    progress("test_jet_nozzle_cfl_ext_mode 40x40x32")
+    n, nzc = 40, 32
+    P = host_array(3, n, n, nzc)
+    P[0] = 1.0
+    P[1] = 1e-4 / (G - 1)
+    dx = 1 / n
+    it = uniform_stream(42)
+    phis = [2 * math.pi * next(it) for _ in range(8)]
+    noz = (8 * dx, 0.01, 1e-4 / (G - 1), math.sqrt(99.0), 0.01)
+    bc = ["outflow"] * 4 + ["nozzle", "outflow"]
+    with DeviceGrid(3, n, n, nzc, G) as dev:
+        dev.upload("P", P)
+        dev.set_nozzle(noz[0], dx, dx, *noz[1:])
+        dev.fill_ghosts(bc)
+        dev.prim_to_cons()
+        dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50,
+                     tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx, bc=bc)
+        tg = sum(dev.step(phi=phis[s], want_dt=True) for s in range(6))
+        Pg = dev.download("P")
+        nbad, _ = dev.counters()
+    Po = to_oracle3(P)
+    mode = oracle.make_mode(ref=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50,
+                            tol=1e-10, gamma=G, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx)
+    bco, keep = oracle.make_bc3([2, 2, 2, 2, 3, 2], dx=dx, dy=dx, nozzle=noz + (0.0,), phi_seq=phis)
+    nbo, to, _ = oracle.run3(Po, mode, bco, 6)
+    Po = from_oracle3(Po)
+    errs = per_var_rel_l1(interior(Pg), interior(Po))
+    progress(f"  per-var rel L1 {['%.2e' % e for e in errs]}, t {tg!r} vs {to!r}, max u_z {Pg[4].max():.3f}")
+    assert nbad == nbo == 0
+    assert max(errs) <= TOL
+    assert abs(tg - to) <= 1e-14 * to
+    assert Pg[4, 3:-3, 3:-3, 3].max() > 1.0  # the jet has entered the grid

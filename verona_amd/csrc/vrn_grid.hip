@@ -551,9 +551,11 @@ __global__ __launch_bounds__(256) void k_rate(Geo G, const double *P, double g, 
     rate_reduce(st, r);
 }
 
-__global__ void k_finalize_dt(DevState *st, double cfl)
+// dt = cfl / max(interior rate, boundary inflow rate); resets the accumulator.
+__global__ void k_finalize_dt(DevState *st, double cfl, double extra_rate)
 {
     double r = __longlong_as_double((long long)st->rate_bits);
+    if (extra_rate > r) r = extra_rate;
     st->dt_prev = st->dt;
     st->dt = cfl / r;
     st->rate_bits = 0ull;
@@ -1283,9 +1285,20 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
                 hipLaunchKernelGGL(k_rate<5>, dim3((G.NX - 6 + 63) / 64, (G.NY - 6 + 3) / 4, G.NZ - 6), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0 / p->dz, c->st);
             else
                 hipLaunchKernelGGL(k_rate<4>, dim3((G.NX - 6 + 63) / 64, (G.NY - 6 + 3) / 4, 1), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0, c->st);
-            hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->s, c->st, p->cfl);
             c->first_cfl = false;
         }
+        // the rate of the cells on the path is in rate_bits (from the last
+        // stage of the previous step, or from k_rate); the nozzle's inflow
+        // state enters through the ghost faces and is added here.
+        double extra = 0.0;
+        if (p->bc[4] == VRN_BC_NOZZLE) {
+            GhostArgs gn{};
+            fill_nozzle_args(c, gn, p->nozzle_phi);
+            const double Pz[5] = {gn.noz_rho, gn.noz_u, gn.noz_ux, gn.noz_uy, gn.noz_uz};
+            const double inv[3] = {1.0 / p->dx, 1.0 / p->dy, 1.0 / p->dz};
+            extra = cell_rate<5>(Pz, c->gamma, inv);
+        }
+        hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->s, c->st, p->cfl, extra);
     }
     c->dxs[0] = p->dx;
     c->dxs[1] = p->dy;
@@ -1361,16 +1374,11 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
         HIPCHK(hipGetLastError());
         std::swap(c->P, c->Pn);
     }
-    if (cflm) {
-        hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->s, c->st, p->cfl);
-        HIPCHK(hipGetLastError());
-    }
     if (dt_used) {
         if (cflm) {
-            // k_finalize_dt moved this step's dt to dt_prev and stored the next one
             HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->s));
             HIPCHK(hipStreamSynchronize(c->s));
-            *dt_used = c->st_host->dt_prev;
+            *dt_used = c->st_host->dt;
         } else {
             *dt_used = p->dt;
         }

@@ -10,13 +10,13 @@
 namespace vrn {
 
 // Julia Base max/min for IEEE floats: NaN-propagating, -0 < +0.
-template <class T> __device__ __forceinline__ T jmax(T x, T y)
+template <class T> __host__ __device__ __forceinline__ T jmax(T x, T y)
 {
     T d = x - y;
     T am = __builtin_signbit(d) ? y : x;
     return (__builtin_isnan(x) || __builtin_isnan(y)) ? d : am;
 }
-template <class T> __device__ __forceinline__ T jmin(T x, T y)
+template <class T> __host__ __device__ __forceinline__ T jmin(T x, T y)
 {
     T d = x - y;
     T am = __builtin_signbit(d) ? x : y;
@@ -29,10 +29,10 @@ template <class T> __device__ __forceinline__ T jsign(T x)
 }
 template <class T> __device__ __forceinline__ T sq(T x) { return x * x; }
 
-__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
-__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
-__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+__host__ __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__host__ __device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__host__ __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+__host__ __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 
 enum { REC_MINMOD = 0, REC_PPM = 1, REC_WENOZ = 2 };
 enum { RIEM_HLL = 0, RIEM_HLLC = 1 };
@@ -168,7 +168,7 @@ __device__ __forceinline__ void recon(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T 
 // 3dim/structs.jl:84-142 (NV = 5) and 2dim/structs.jl:79-119 (NV = 4):
 // lor = sqrt(sum u_i^2 + 1), w = gamma*u + rho,
 // U = (lor rho, (gamma-1)u - lor^2 w, u_i lor w), F^d as function_PtoF{x,y,z}.
-template <int NV, class T> __device__ __forceinline__ T lorentz(const T *P)
+template <int NV, class T> __host__ __device__ __forceinline__ T lorentz(const T *P)
 {
     if constexpr (NV == 5)
         return dsqrt(P[2] * P[2] + P[3] * P[3] + P[4] * P[4] + T(1));
@@ -213,7 +213,7 @@ __device__ __forceinline__ void p2f(const T *P, T *F, T g, T gam)
 }
 
 // eos.jl:15-17
-template <class T> __device__ __forceinline__ T sound_speed(T rho, T u, T g)
+template <class T> __host__ __device__ __forceinline__ T sound_speed(T rho, T u, T g)
 {
     return dsqrt((g * (g - T(1)) * u) / (rho + g * u));
 }
@@ -221,7 +221,7 @@ template <class T> __device__ __forceinline__ T sound_speed(T rho, T u, T g)
 // 3dim/algos.jl:146-157: sigma = c^2/(lor^2 (1-c^2)),
 // lam+- = (v +- sqrt(sigma (1 - v^2 + sigma)))/(1 + sigma)
 template <class T>
-__device__ __forceinline__ void lambdas(T v, T cs, T lor2, T &lp, T &lm)
+__host__ __device__ __forceinline__ void lambdas(T v, T cs, T lor2, T &lp, T &lm)
 {
     T sig = cs * cs / (lor2 * (T(1) - cs * cs));
     T rt = dsqrt(sig * (T(1) - v * v + sig));
@@ -321,7 +321,7 @@ __device__ __forceinline__ void face_flux(const T *PL, const T *PR, T g, T *F)
 // Cell-centred signal-speed rate sum_d max(|lam+_d|, |lam-_d|)/dx_d used by
 // the CFL reduction (wave speeds as in 3dim/algos.jl:130-157).
 template <int NV>
-__device__ __forceinline__ double cell_rate(const double *P, double g, const double *inv_dx)
+__host__ __device__ __forceinline__ double cell_rate(const double *P, double g, const double *inv_dx)
 {
     double lor = lorentz<NV>(P);
     double cs = sound_speed(P[0], P[1], g);
