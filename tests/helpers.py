@@ -71,8 +71,9 @@ def state2(nx, ny, seed=0, jump=True, umax=0.5):
 
 
 def to_oracle3(P):
-    """Verona (5, NX, NY, NZ) Fortran array -> oracle's (NZ, NY, NX, 5) C array (same bytes)."""
-    return np.ascontiguousarray(np.asfortranarray(P).T)
+    """Verona (5, NX, NY, NZ) Fortran array -> a fresh oracle (NZ, NY, NX, 5) C array
+    (same byte order; always a copy, so in-place oracle runs leave P alone)."""
+    return np.array(np.asfortranarray(P).T, order="C", copy=True)
 
 
 def from_oracle3(A):

@@ -268,4 +268,4 @@ def test_jet_nozzle_cfl_ext_mode(oracle):
     assert nbad == nbo == 0
     assert max(errs) <= TOL
     assert abs(tg - to) <= 1e-14 * to
-    assert Pg[4, 3:-3, 3:-3, 3].max() > 1.0  # the jet has entered the grid
+    assert Pg[4, 3:-3, 3:-3, 3].max() > 0.5  # the jet has entered the grid

@@ -1,0 +1,376 @@
+"""Pins for the CPU oracle (CPU only).
+
+The reference ships no tests, golden vectors or fixtures for this path, and
+its language (Julia) is absent, so every oracle function is pinned by checks
+that a plausible restatement mistake (a dropped term, a sign, a swapped
+index or operand) would fail: closed forms, textbook special cases, exact
+symmetries, conservation, the exact Riemann solution, brute-force roots and
+convergence orders.
+"""
+import math
+
+import numpy as np
+import pytest
+
+G3 = 4.0 / 3.0
+rng = np.random.default_rng(20261202)
+
+
+# ----------------------------------------------------------------- reconstruction
+def test_wenoz_constant_and_linear(oracle):
+    # constants come back to rounding (7c and 11c round); on linear data every
+    # beta is b^2, tau5 = 0, the linear weights apply and the faces are q -+ b/2
+    # (fluxlimiter.jl:19-71)
+    c = rng.uniform(-5, 5, 50)
+    lo, hi = oracle.wenoz(np.repeat(c[:, None], 5, axis=1))
+    assert np.allclose(lo, c, rtol=1e-15, atol=0) and np.allclose(hi, c, rtol=1e-15, atol=0)
+    a = rng.uniform(-3, 3, 200)
+    b = rng.uniform(-2, 2, 200)
+    q = a[:, None] + b[:, None] * np.arange(-2, 3)[None, :]
+    lo, hi = oracle.wenoz(q)
+    assert np.allclose(lo, a - b / 2, rtol=0, atol=1e-13 * (1 + np.abs(a)).max())
+    assert np.allclose(hi, a + b / 2, rtol=0, atol=1e-13 * (1 + np.abs(a)).max())
+
+
+def test_wenoz_mirror_symmetry(oracle):
+    # WENOZ(a,b,c,d,e) = swap(WENOZ(e,d,c,b,a)) bit for bit
+    q = rng.uniform(-1, 1, (500, 5)) * rng.uniform(0.1, 100, (500, 1))
+    lo, hi = oracle.wenoz(q)
+    lo2, hi2 = oracle.wenoz(q[:, ::-1])
+    assert np.array_equal(lo, hi2) and np.array_equal(hi, lo2)
+
+
+def test_wenoz_fifth_order(oracle):
+    # face values from exact cell averages of sin(2 pi x): error ~ h^5
+    errs = []
+    for n in (20, 40, 80):
+        h = 1.0 / n
+        x = (np.arange(n) + 0.5) * h
+        avg = lambda xc: (np.cos(2 * np.pi * (xc - h / 2)) - np.cos(2 * np.pi * (xc + h / 2))) / (2 * np.pi * h)
+        idx = np.arange(n)
+        q = np.stack([avg(x + k * h) for k in range(-2, 3)], axis=1)
+        lo, hi = oracle.wenoz(q)
+        errs.append(np.abs(hi - np.sin(2 * np.pi * (x + h / 2))).max())
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert min(rates) > 4.5, rates
+
+
+def test_ppm_constant_linear_symmetry_monotone(oracle):
+    c = rng.uniform(-5, 5, 50)
+    lo, hi = oracle.ppm(np.repeat(c[:, None], 5, axis=1))
+    assert np.allclose(lo, c, rtol=1e-15, atol=0) and np.allclose(hi, c, rtol=1e-15, atol=0)
+    a = rng.uniform(-3, 3, 200)
+    b = rng.uniform(-2, 2, 200)
+    q = a[:, None] + b[:, None] * np.arange(-2, 3)[None, :]
+    lo, hi = oracle.ppm(q)
+    assert np.allclose(lo, a - b / 2, atol=1e-12) and np.allclose(hi, a + b / 2, atol=1e-12)
+    qr = rng.uniform(-1, 1, (500, 5))
+    lo, hi = oracle.ppm(qr)
+    lo2, hi2 = oracle.ppm(qr[:, ::-1])
+    assert np.allclose(lo, hi2, atol=1e-14) and np.allclose(hi, lo2, atol=1e-14)
+    # extremum-preserving limiter: a step (monotone data) gives no new extrema
+    s = np.array([[0.0, 0.0, 0.0, 1.0, 1.0], [0.0, 0.0, 1.0, 1.0, 1.0], [0.0, 1.0, 1.0, 1.0, 1.0]])
+    lo, hi = oracle.ppm(s)
+    assert np.all((lo >= 0) & (lo <= 1) & (hi >= 0) & (hi <= 1))
+
+
+def test_minmod_textbook(oracle):
+    q = rng.uniform(-1, 1, (300, 3))
+    lo, hi = oracle.minmod(q)
+    sp, sm = q[:, 2] - q[:, 1], q[:, 1] - q[:, 0]
+    slope = np.where(sp * sm > 0, np.sign(sp) * np.minimum(abs(sp), abs(sm)), 0.0)
+    assert np.allclose(hi - q[:, 1], slope / 2, atol=1e-15)
+    assert np.allclose(q[:, 1] - lo, slope / 2, atol=1e-15)
+    # extremum -> zero slope, Julia sign(0) = 0 gives zero slope on a flat side
+    lo, hi = oracle.minmod(np.array([[0.0, 1.0, 0.0], [1.0, 1.0, 2.0]]))
+    assert np.array_equal(lo, [1.0, 1.0]) and np.array_equal(hi, [1.0, 1.0])
+
+
+# ----------------------------------------------------------------- conversions
+def rand_prims(n, umax=3.0):
+    P = np.empty((n, 5))
+    P[:, 0] = 10 ** rng.uniform(-3, 1, n)
+    P[:, 1] = 10 ** rng.uniform(-4, 1, n)
+    P[:, 2:] = rng.uniform(-umax, umax, (n, 3))
+    return P
+
+
+def test_p2u_p2f_closed_form(oracle):
+    P = rand_prims(400)
+    rho, u, ui = P[:, 0], P[:, 1], P[:, 2:]
+    p = (G3 - 1) * u
+    h = 1 + G3 * u / rho
+    W = np.sqrt(1 + (ui ** 2).sum(1))
+    U = oracle.p2u3(P, G3)
+    assert np.allclose(U[:, 0], rho * W, rtol=1e-14)
+    assert np.allclose(U[:, 1], p - rho * h * W ** 2, rtol=1e-13)  # T^t_t (negative)
+    assert np.allclose(U[:, 2:], (rho * h * W)[:, None] * ui, rtol=1e-13)
+    for d in range(3):
+        F = oracle.p2f3(P, G3, d)
+        assert np.allclose(F[:, 0], rho * ui[:, d], rtol=1e-14)
+        assert np.allclose(F[:, 1], -rho * h * W * ui[:, d], rtol=1e-13)
+        for k in range(3):
+            ref = rho * h * ui[:, d] * ui[:, k] + (p if k == d else 0.0)
+            assert np.allclose(F[:, 2 + k], ref, rtol=1e-13, atol=1e-300)
+
+
+def test_wave_speeds_are_srhd_eigenvalues(oracle):
+    # lam+- of 3dim/algos.jl:146-157 equal the textbook acoustic eigenvalues
+    # [v_d(1-c^2) +- c sqrt((1-v^2)(1 - v^2 c^2 - v_d^2 (1-c^2)))] / (1 - v^2 c^2)
+    P = rand_prims(300, umax=2.0)
+    for d in range(3):
+        # identical left/right states: the face solver returns Cmax = lam+, Cmin = -lam-
+        _, cmax, cmin = oracle.face_flux3(P, P, G3, d)
+        W = np.sqrt(1 + (P[:, 2:] ** 2).sum(1))
+        v = P[:, 2:] / W[:, None]
+        v2 = (v ** 2).sum(1)
+        vd = v[:, d]
+        p = (G3 - 1) * P[:, 1]
+        c = np.sqrt(G3 * p / (P[:, 0] + G3 * P[:, 1]))
+        root = c * np.sqrt((1 - v2) * (1 - v2 * c * c - vd * vd * (1 - c * c)))
+        lp = (vd * (1 - c * c) + root) / (1 - v2 * c * c)
+        lm = (vd * (1 - c * c) - root) / (1 - v2 * c * c)
+        assert np.allclose(cmax, lp, rtol=1e-11, atol=1e-14)
+        assert np.allclose(cmin, -lm, rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("riemann", ["hll", "hllc"])
+def test_riemann_consistency_and_upwinding(oracle, riemann):
+    P = rand_prims(200)
+    for d in range(3):
+        F, cmax, cmin = oracle.face_flux3(P, P, G3, d, riemann)
+        Fe = oracle.p2f3(P, G3, d)
+        assert np.allclose(F, Fe, rtol=1e-10, atol=1e-12 * np.abs(Fe).max())
+    # supersonic to the right: every wave moves right, flux = F(left)
+    PL = rand_prims(100, umax=0.1)
+    PL[:, 2] = 30.0
+    PL[:, 1] = 1e-3 * PL[:, 0]
+    PR = PL.copy()
+    PR[:, 0] *= 0.5
+    F, cmax, cmin = oracle.face_flux3(PL, PR, G3, 0, riemann)
+    assert np.all(cmin < 0)
+    assert np.array_equal(F, oracle.p2f3(PL, G3, 0))
+
+
+def test_hllc_resolves_stationary_contact(oracle):
+    # a contact at rest with equal pressure: HLLC gives zero mass and energy
+    # flux and the pressure flux exactly; HLL diffuses mass across it
+    n = 50
+    PL = np.zeros((n, 5))
+    PR = np.zeros((n, 5))
+    PL[:, 0] = 10 ** rng.uniform(-1, 1, n)
+    PR[:, 0] = 10 ** rng.uniform(-1, 1, n)
+    p = 10 ** rng.uniform(-2, 1, n)
+    PL[:, 1] = PR[:, 1] = p / (G3 - 1)
+    Fc, _, _ = oracle.face_flux3(PL, PR, G3, 0, "hllc")
+    Fh, _, _ = oracle.face_flux3(PL, PR, G3, 0, "hll")
+    assert np.allclose(Fc[:, 0], 0, atol=1e-14 * PL[:, 0].max())
+    assert np.allclose(Fc[:, 1], 0, atol=1e-13)
+    assert np.allclose(Fc[:, 2], p, rtol=1e-12)
+    assert np.all(np.abs(Fh[:, 0]) > 1e-6 * np.abs(PL[:, 0] - PR[:, 0]))
+
+
+# ----------------------------------------------------------------- con2prim
+def test_c2p3_round_trip(oracle):
+    P = rand_prims(2000, umax=5.0)  # Gamma up to ~9
+    U = oracle.p2u3(P, G3)
+    guess = P * (1 + 0.05 * rng.standard_normal(P.shape))
+    guess[:, :2] = np.abs(guess[:, :2])
+    Pr, bad, meth, its = oracle.c2p3_batch(U, guess, G3, 50, 1e-12)
+    assert bad == 0
+    ok = (Pr[:, 0] > 1e-8) & (Pr[:, 1] > 1e-8)
+    assert np.allclose(Pr[ok], P[ok], rtol=1e-8, atol=1e-10)
+
+
+def test_c2p3_1dw_residual_and_root(oracle):
+    """The 1DW residual of 3dim/structs.jl:315-317 vanishes at the true W and
+    the oracle's W matches a 50-digit mpmath root of it."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 50
+    P = rand_prims(20, umax=2.0)
+    U = oracle.p2u3(P, G3)
+    for k in range(len(P)):
+        D, Qt = U[k, 0], U[k, 1]
+        S2 = (U[k, 2:] ** 2).sum()
+        c = mpmath.mpf(G3 - 1) / mpmath.mpf(G3)
+        f = lambda W: Qt + W - c * (W * (1 - S2 / W ** 2) - D * mpmath.sqrt(1 - S2 / W ** 2))
+        W0 = mpmath.mpf(P[k, 0] + G3 * P[k, 1]) * (1 + (P[k, 2:] ** 2).sum())
+        Wt = mpmath.findroot(f, W0)
+        assert abs(f(W0)) < 1e-12 * W0
+        Pr, m, _ = oracle.c2p3_cell(U[k], P[k] * 1.01, G3, 50, 1e-13)
+        Wr = (Pr[0] + G3 * Pr[1]) * (1 + (Pr[2:] ** 2).sum())
+        assert m == 1
+        assert abs(Wr - float(Wt)) <= 1e-11 * float(Wt)
+
+
+def test_c2p3_forced_failure_and_cascade(oracle):
+    P = rand_prims(50)
+    U = oracle.p2u3(P, G3)
+    Pr, bad, meth, _ = oracle.c2p3_batch(U, P, G3, 0, 1e-10)
+    assert bad == 50 and np.all(np.isnan(Pr)) and np.all(meth == 0)
+    # a superluminal state with no physical root ends non-finite and is counted
+    Pr, bad, meth, _ = oracle.c2p3_batch(np.array([[1.0, -1.1, 5.0, 0, 0]]), np.array([[1.0, 1, 0, 0, 0]]), G3, 50, 1e-10)
+    assert bad == 1
+    # Lorentz-factor cap (3dim/structs.jl:600-608): gamma = min(1/sqrt(1-v^2), 50)
+    # and u^i = gamma Q_i / W, so |u| = gamma v stays below 50
+    Pf = np.array([[1.0, 1.0, 99.0, 0, 0]])
+    Uf = oracle.p2u3(Pf, G3)
+    Pr, bad, meth, _ = oracle.c2p3_batch(Uf, Pf * 1.001, G3, 50, 1e-10)
+    umag = np.sqrt((Pr[0, 2:] ** 2).sum())
+    assert bad == 0 and 49.9 < umag <= 50.0
+
+
+def test_c2p3_rotation_and_scaling(oracle):
+    # the map depends on |S| only and is homogeneous of degree one in (D, tau, S)
+    P = rand_prims(200, umax=2.0)
+    U = oracle.p2u3(P, G3)
+    Pr, _, _, _ = oracle.c2p3_batch(U, P, G3, 50, 1e-13)
+    Ur = U.copy()
+    Ur[:, 2:] = U[:, [3, 4, 2]]
+    Pr2, _, _, _ = oracle.c2p3_batch(Ur, P[:, [0, 1, 3, 4, 2]], G3, 50, 1e-13)
+    assert np.allclose(Pr2[:, [0, 1, 4, 2, 3]], Pr, rtol=1e-9)
+    Us = 3.0 * U
+    Ps, _, _, _ = oracle.c2p3_batch(Us, P * [3, 3, 1, 1, 1], G3, 50, 1e-13)
+    ok = (Pr[:, 0] > 1e-7) & (Pr[:, 1] > 1e-7)
+    assert np.allclose(Ps[ok, :2], 3 * Pr[ok, :2], rtol=1e-9)
+    assert np.allclose(Ps[ok, 2:], Pr[ok, 2:], rtol=1e-9, atol=1e-12)
+
+
+def test_c2p2_and_c2p1_round_trip(oracle):
+    from helpers import state2, to_oracle2
+    P = state2(24, 20, seed=3)
+    Po = to_oracle2(P)
+    U = oracle.ptou2(Po, G3)
+    guess = Po * (1 + 0.02 * rng.standard_normal(Po.shape))
+    ns = oracle.c2p2(U, guess, G3, 50, 1e-13)
+    assert ns == 0
+    assert np.allclose(guess[3:-3, 3:-3], Po[3:-3, 3:-3], rtol=1e-9, atol=1e-12)
+    g1 = 5.0 / 3.0
+    n = 300
+    P1 = np.stack([10 ** rng.uniform(-2, 1, n), 10 ** rng.uniform(-3, 1, n), rng.uniform(-3, 3, n)])
+    W = np.sqrt(1 + P1[2] ** 2)
+    p = (g1 - 1) * P1[1]
+    w = P1[0] + P1[1] + p
+    U1 = np.stack([P1[0] * W, -w * W ** 2 + p, w * W * P1[2]])
+    guess1 = np.ascontiguousarray(P1 * (1 + 0.02 * rng.standard_normal(P1.shape)))
+    oracle.c2p1(np.ascontiguousarray(U1), guess1, g1, 50, 1e-13)
+    assert np.allclose(guess1, P1, rtol=1e-9, atol=1e-12)
+
+
+# ----------------------------------------------------------------- grid-level invariants
+def test_x_only_data_reduces_to_1d(oracle):
+    """3D fluxes of data that vary along x only: transverse flux differences
+    vanish exactly and the x-flux equals the 1D HLL flux of the same face
+    states up to the 1D path's different (w = u + p + rho) association."""
+    from helpers import state3, to_oracle3
+    P = state3(24, 8, 8, seed=2)
+    P[:] = P[:, :, 5:6, 5:6]
+    P[3:] = 0.0
+    Po = to_oracle3(P)
+    Fy = oracle.fluxes3(Po, G3, 1e-7, 1)
+    Fz = oracle.fluxes3(Po, G3, 1e-7, 2)
+    sl = (slice(3, -3), slice(3, -3), slice(3, -3))
+    assert np.array_equal(Fy[3:-3, 3:-3, 3:-3] - Fy[3:-3, 2:-4, 3:-3], np.zeros_like(Fy[sl]))
+    assert np.array_equal(Fz[3:-3, 3:-3, 3:-3] - Fz[2:-4, 3:-3, 3:-3], np.zeros_like(Fz[sl]))
+    Fx = oracle.fluxes3(Po, G3, 1e-7, 0)
+    row = Po[5, 5]  # (NX, 5)
+    lo, hi = oracle.wenoz(np.stack([row[k - 2:k + 3, v] for k in range(2, 28) for v in range(5)]).reshape(-1, 5))
+    lo = lo.reshape(26, 5)
+    hi = hi.reshape(26, 5)
+    PL = np.maximum(hi[:-1], [1e-7, 1e-7, -np.inf, -np.inf, -np.inf])
+    PR = np.maximum(lo[1:], [1e-7, 1e-7, -np.inf, -np.inf, -np.inf])
+    F1 = oracle.face_flux1(PL[:, :3], PR[:, :3], G3)
+    assert np.allclose(Fx[5, 5, 2:27, [0, 1, 2]].T, F1, rtol=1e-13, atol=1e-15)
+
+
+def test_periodic_conservation_3d(oracle):
+    # telescoping flux differences: the interior sums of D, T^t_t and T^t_i are
+    # unchanged by a step on a periodic grid (to rounding)
+    from helpers import state3, to_oracle3
+    P = state3(20, 16, 12, seed=4)
+    Po = to_oracle3(P)
+    oracle.fill_periodic3(Po)
+    U0 = oracle.ptou3(Po, G3)[3:-3, 3:-3, 3:-3].sum(axis=(0, 1, 2))
+    dx, dy, dz = 1 / 20, 1 / 16, 1 / 12
+    for integ, riem in (("rk2", "hll"), ("rk3", "hllc")):
+        Pc = Po.copy()
+        m = oracle.make_mode(ref=integ == "rk2", riemann=riem, integrator=integ, gamma=G3, floor=1e-7,
+                             dt=0.3 / (1 / dx + 1 / dy + 1 / dz), dx=dx, dy=dy, dz=dz, n_iter=50, tol=1e-13)
+        bc, _ = oracle.make_bc3([1] * 6)
+        oracle.run3(Pc, m, bc, 3)
+        U1 = oracle.ptou3(Pc, G3)[3:-3, 3:-3, 3:-3].sum(axis=(0, 1, 2))
+        assert np.allclose(U1, U0, rtol=1e-10, atol=1e-10 * np.abs(U0).max())
+
+
+def test_axis_permutation_symmetry_3d(oracle):
+    # rotating the state x -> y -> z rotates the result of a step
+    from helpers import state3, to_oracle3
+    P = state3(16, 16, 16, seed=5)
+    Pr = np.asfortranarray(np.stack([P[0], P[1], P[4], P[2], P[3]]).transpose(0, 3, 1, 2))
+    outs = []
+    for Q in (P, Pr):
+        Qo = to_oracle3(Q)
+        m = oracle.make_mode(ref=True, gamma=G3, floor=1e-7, dt=0.4 / 48 * 1.0, dx=1 / 16, dy=1 / 16,
+                             dz=1 / 16, n_iter=50, tol=1e-12)
+        bc, _ = oracle.make_bc3([1] * 6)
+        oracle.run3(Qo, m, bc, 2)
+        outs.append(np.asfortranarray(Qo.T))
+    back = np.stack([outs[1][0], outs[1][1], outs[1][3], outs[1][4], outs[1][2]]).transpose(0, 2, 3, 1)
+    assert np.allclose(back[:, 3:-3, 3:-3, 3:-3], outs[0][:, 3:-3, 3:-3, 3:-3], rtol=1e-12, atol=1e-14)
+
+
+# ----------------------------------------------------------------- 1D shock tube vs exact solution
+def _tube(N, rec="minmod", riemann="hll"):
+    g = 5.0 / 3.0
+    P = np.zeros((3, N))
+    h = N // 2
+    P[0, :h], P[1, :h] = 10.0, 13.33 / (g - 1)
+    P[0, h:], P[1, h:] = 1.0, 1e-6 / (g - 1)
+    return P, g
+
+
+@pytest.mark.parametrize("riemann", ["hll", "hllc"])
+def test_shock_tube_converges_to_exact_solution(oracle, riemann):
+    """Marti & Mueller problem 1 (the 1D shock-tube workload): L1 density
+    error against the exact solution at t = 0.4 falls with resolution and is
+    small at N = 1024."""
+    import exact_riemann as ER
+    errs = []
+    for N in (256, 1024):
+        P, g = _tube(N, riemann=riemann)
+        dx = 1.0 / N
+        steps, t = oracle.run1(P, 0.4 * dx, dx, 0.4, g, "minmod", riemann, 50, 1e-10)
+        x = (np.arange(N) + 0.5) * dx
+        ex = np.array(ER.sample((10.0, 13.33, 0.0), (1.0, 1e-6, 0.0), g, x, t))
+        errs.append(np.abs(P[0] - ex[:, 0]).mean())
+        if N == 1024:
+            # star state on the plateau between the rarefaction tail and the contact
+            pst = (g - 1) * P[1]
+            vst = P[2] / np.sqrt(1 + P[2] ** 2)
+            sel = (x > 0.58) & (x < 0.72)
+            assert abs(np.median(pst[sel]) - 1.44769) < 5e-3
+            assert abs(np.median(vst[sel]) - 0.71399) < 2e-3
+    assert errs[1] < 0.6 * errs[0]
+    assert errs[1] < 0.06
+
+
+def test_shock_tube_frozen_edges(oracle):
+    # 1dim/algos.jl:173-177 update i = 2..N-2 (1-based): cells 1, N-1, N never change
+    P, g = _tube(128)
+    P[:, -2:] = [[1.5, 1.7], [1e-3, 2e-3], [0.1, -0.1]]
+    P0 = P.copy()
+    oracle.run1(P, 0.4 / 128, 1 / 128, 1.0, g, "ppm", "hll", 50, 1e-10, max_steps=20)
+    assert np.array_equal(P[:, 0], P0[:, 0])
+    assert np.array_equal(P[:, -2:], P0[:, -2:])
+
+
+def test_time_accumulation_step_count(oracle):
+    # while t < T: t += dt with FP64 accumulation decides 1024 vs 1025 steps
+    P, g = _tube(64)
+    dx = 1 / 1024
+    steps, t = oracle.run1(P, 0.4 * dx, dx, 0.4, g, "minmod", "hll", 5, 1e-8)
+    tt, k = 0.0, 0
+    while tt < 0.4:
+        tt += 0.4 * dx
+        k += 1
+    assert steps == k and t == tt

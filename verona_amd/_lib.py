@@ -8,7 +8,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libverona_hip.so")
+LIB_PATH = os.environ.get("VERONA_HIP_LIB", os.path.join(_HERE, "libverona_hip.so"))
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "verona_hip.h")
 
 VRN_OK = 0

@@ -36,6 +36,10 @@
 #include <type_traits>
 #include <vector>
 
+#ifndef VRN_MARCH_MIN_WAVES
+#define VRN_MARCH_MIN_WAVES 2  // measured: 2 waves/SIMD beats the unbounded allocation
+#endif
+
 namespace vrn {
 
 constexpr int XOFF = 13;
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
 // One lane per interior (x, o) column, marching along DIM over the segment
 // [m0, m1) of interior cells with the 5-cell window in registers.
 template <int NV, int DIM, class R, int REC, int RIEM, bool LAST>
-__global__ __launch_bounds__(256) void k_march(SweepArgs a)
+__global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 {
     const Geo G = a.geo;
     const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
@@ -531,22 +535,24 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a)
 }
 
 // ------------------------------------------------------------ CFL
+// Max signal rate over interior cells: grid-stride loop, one atomic per wave.
 template <int NV>
 __global__ __launch_bounds__(256) void k_rate(Geo G, const double *P, double g, double ix,
                                                double iy, double iz, DevState *st)
 {
-    const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = 3 + blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = NV == 5 ? 3 + blockIdx.z : 0;
-    if (j >= G.NY - 3) return;
+    const int ni = G.NX - 6, nj = G.NY - 6, nk = NV == 5 ? G.NZ - 6 : 1;
+    const long n = (long)ni * nj * nk;
+    const double inv[3] = {ix, iy, iz};
     double r = 0.0;
-    if (x < G.NX - 3) {
-        const long c = gidx(G, x, j, k);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = 3 + (int)(t % ni);
+        const long q = t / ni;
+        const long c = gidx(G, i, 3 + (int)(q % nj), NV == 5 ? 3 + (int)(q / nj) : 0);
         double p[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) p[v] = P[v * G.ps + c];
-        const double inv[3] = {ix, iy, iz};
-        r = cell_rate<NV>(p, g, inv);
+        double cr = cell_rate<NV>(p, g, inv);
+        if (cr > r) r = cr;
     }
     rate_reduce(st, r);
 }
@@ -1282,9 +1288,9 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
         if (c->first_cfl) {
             Timed tm(c, KN_CFL);
             if (c->NV == 5)
-                hipLaunchKernelGGL(k_rate<5>, dim3((G.NX - 6 + 63) / 64, (G.NY - 6 + 3) / 4, G.NZ - 6), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0 / p->dz, c->st);
+                hipLaunchKernelGGL(k_rate<5>, dim3(2048), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0 / p->dz, c->st);
             else
-                hipLaunchKernelGGL(k_rate<4>, dim3((G.NX - 6 + 63) / 64, (G.NY - 6 + 3) / 4, 1), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0, c->st);
+                hipLaunchKernelGGL(k_rate<4>, dim3(2048), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0, c->st);
             c->first_cfl = false;
         }
         // the rate of the cells on the path is in rate_bits (from the last
