@@ -164,6 +164,35 @@ int vrn_timing_enable(vrn_ctx *ctx, int enable);
 int vrn_timing_report(vrn_ctx *ctx, char *names, int name_len, int64_t *counts, double *mean_ms,
                       int cap, int *n_out);
 
+
+/* ------------------------------------------------------------------------
+ * Batched 1D shock tubes: many independent Verona1D.HARM_HLL problems
+ * (src/1dim/algos.jl:58-315, SoA ParVector1D without ghosts, the 3x3 Newton
+ * con2prim of src/1dim/structs.jl:78-105) advanced together.  The reference
+ * has no batched form; each tube reproduces one HARM_HLL run step for step.
+ * ------------------------------------------------------------------------ */
+typedef struct vrn_tubes vrn_tubes;
+
+/* ntubes tubes of lengths[t] >= 8 cells, adiabatic index gamma. */
+int vrn_tubes_create(int ntubes, const int64_t *lengths, double gamma, vrn_tubes **out);
+int vrn_tubes_destroy(vrn_tubes *tb);
+const char *vrn_tubes_last_error(void);
+/* Concatenated SoA primitives (tubes back to back) and per-tube dt, dx:
+ * the arguments P, dt, dx of HARM_HLL; U = PtoU(P) is formed on the device
+ * (src/1dim/algos.jl:82). */
+int vrn_tubes_upload(vrn_tubes *tb, const double *rho, const double *u, const double *ux,
+                     const double *dt, const double *dx);
+int vrn_tubes_download(vrn_tubes *tb, double *rho, double *u, double *ux);
+/* rec (VRN_REC_*), riemann (VRN_RIEMANN_*), n_iter and tol (kwargs of HARM_HLL). */
+int vrn_tubes_set_method(vrn_tubes *tb, int rec, int riemann, int n_iter, double tol);
+/* Advance all tubes by nsteps midpoint-RK2 steps with `grid` persistent
+ * workgroups of `block` threads (multiple of 64); use_graph replays a
+ * captured hipGraph.  *ms receives the device time.  Blocks. */
+int vrn_tubes_run(vrn_tubes *tb, int nsteps, int block, int grid, int use_graph, double *ms);
+/* restore = 0 saves (P, U), restore = 1 puts them back (used by the autotuner). */
+int vrn_tubes_checkpoint(vrn_tubes *tb, int restore);
+int vrn_tubes_info(vrn_tubes *tb, int64_t *ncell, int *cu_count);
+
 #ifdef __cplusplus
 }
 #endif

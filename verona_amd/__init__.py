@@ -5,9 +5,10 @@ C ABI of include/verona_hip.h); this package is the host-side mirror of
 Verona's Julia API (Verona.EosTypes, Verona2D, Verona3D).
 """
 from . import eos as EosTypes
+from . import verona1d as Verona1D
 from . import verona2d as Verona2D
 from . import verona3d as Verona3D
 from .bc import Boundary, Cart
 from .device import DeviceGrid, host_array
 
-__all__ = ["EosTypes", "Verona2D", "Verona3D", "Boundary", "Cart", "DeviceGrid", "host_array"]
+__all__ = ["EosTypes", "Verona1D", "Verona2D", "Verona3D", "Boundary", "Cart", "DeviceGrid", "host_array"]
