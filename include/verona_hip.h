@@ -59,8 +59,9 @@ typedef struct vrn_ctx vrn_ctx;
 #define VRN_INT_RK3 1
 /* arithmetic of reconstruction + Riemann solve (con2prim and the conserved
  * accumulation are always FP64) */
-#define VRN_PREC_FP64 0
-#define VRN_PREC_FP32 1
+#define VRN_PREC_FP64 0      /* operation order of the reference, bit-reproducible */
+#define VRN_PREC_FP32 1      /* FP32 reconstruction + Riemann solve */
+#define VRN_PREC_FP64_FAST 2 /* FP64 with FMA contraction and reciprocal division */
 /* boundary kinds, per face */
 #define VRN_BC_NONE 0     /* ghosts left as they are (MPI.PROC_NULL neighbour) */
 #define VRN_BC_PERIODIC 1 /* single-rank periodic halo exchange */

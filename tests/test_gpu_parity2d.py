@@ -87,3 +87,28 @@ def test_fused_step2(oracle, mode):
     errs = per_var_rel_l1(interior(Pg), interior(from_oracle2(Po)))
     progress(f"  per-var rel L1 {['%.2e' % e for e in errs]}")
     assert max(errs) <= TOL
+
+
+def test_fast_fp64_form2(oracle):
+    from verona_amd import DeviceGrid
+    progress("test_fast_fp64_form2")
+    nx, ny = 72, 56
+    P = state2(nx, ny, seed=4)
+    dx, dy = 1 / nx, 1 / ny
+    dt = 0.4 / (1 / dx + 1 / dy)
+    bc = ["periodic"] * 4 + ["none", "none"]
+    with DeviceGrid(2, nx, ny, 1, G) as dev:
+        dev.upload("P", P)
+        dev.fill_ghosts(bc)
+        dev.prim_to_cons()
+        dev.set_step(ref_mode=True, precision="fp64fast", n_iter=50, tol=1e-10, floor=1e-7, dt=dt,
+                     dx=dx, dy=dy, bc=bc)
+        for _ in range(3):
+            dev.step()
+        Pg = dev.download("P")
+    Po = to_oracle2(P)
+    m = oracle.make_mode(ref=True, gamma=G, floor=1e-7, dt=dt, dx=dx, dy=dy, n_iter=50, tol=1e-10)
+    oracle.run2(Po, m, [1, 1, 1, 1], 3)
+    errs = per_var_rel_l1(interior(Pg), interior(from_oracle2(Po)))
+    progress(f"  per-var rel L1 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= TOL

@@ -269,3 +269,59 @@ def test_jet_nozzle_cfl_ext_mode(oracle):
     assert max(errs) <= TOL
     assert abs(tg - to) <= 1e-14 * to
     assert Pg[4, 3:-3, 3:-3, 3].max() > 0.5  # the jet has entered the grid
+
+
+def test_fp32_form_on_jet_has_no_unrecovered_cells(oracle):
+    """The FP32 reconstruction/Riemann form on the jet workload (a flat medium
+    with a relativistic inflow front, where FP32 WENO-Z weights would overflow
+    without rescaling): no unrecovered cells, and its L1 error against the
+    all-FP64 oracle is reported."""
+    import math
+
+    from verona_amd import DeviceGrid, host_array
+    from verona_amd.bc import uniform_stream
+    progress("test_fp32_form_on_jet_has_no_unrecovered_cells 32^3")
+    n = 32
+    P = host_array(3, n, n, n)
+    P[0] = 1.0
+    P[1] = 1e-4 / (G - 1)
+    dx = 1 / n
+    it = uniform_stream(42)
+    phis = [2 * math.pi * next(it) for _ in range(10)]
+    noz = (8 * dx, 0.01, 1e-4 / (G - 1), math.sqrt(99.0), 0.01)
+    bc = ["outflow"] * 4 + ["nozzle", "outflow"]
+    out = {}
+    for prec in ("fp64", "fp32"):
+        with DeviceGrid(3, n, n, n, G) as dev:
+            dev.upload("P", P)
+            dev.set_nozzle(noz[0], dx, dx, *noz[1:])
+            dev.fill_ghosts(bc)
+            dev.prim_to_cons()
+            dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision=prec,
+                         n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx, bc=bc)
+            for s in range(10):
+                dev.step(phi=phis[s])
+            out[prec] = (dev.download("P"), dev.counters()[0])
+    assert out["fp32"][1] == 0 and out["fp64"][1] == 0
+    errs = per_var_rel_l1(interior(out["fp32"][0]), interior(out["fp64"][0]))
+    progress(f"  fp32 vs fp64 per-var rel L1 after 10 steps: {['%.2e' % e for e in errs]}")
+    assert max(errs) < 1e-3
+
+
+@pytest.mark.parametrize("mode", ["ref", "ext"])
+def test_fast_fp64_form_within_tolerance(oracle, mode):
+    """FP64 with FMA contraction and reciprocal division: not bit-identical,
+    but within the FP64 tolerance (relative L1 <= 1e-12 per variable) of the
+    oracle after several steps."""
+    progress(f"test_fast_fp64_form_within_tolerance {mode}")
+    P = state3(40, 24, 20, seed=13)
+    if mode == "ref":
+        Pg, Po, tg, to, nb, nbo = _run_both(oracle, P, 4, precision="fp64fast")
+    else:
+        Pg, Po, tg, to, nb, nbo = _run_both(oracle, P, 3, ref=False, riemann="hllc", integrator="rk3",
+                                            cfl=0.4, precision="fp64fast",
+                                            bc=("outflow",) * 2 + ("periodic",) * 2 + ("outflow",) * 2)
+    errs = per_var_rel_l1(interior(Pg), interior(Po))
+    progress(f"  per-var rel L1 {['%.2e' % e for e in errs]}, bit-exact {np.mean(Pg == Po):.4f}")
+    assert nb == nbo == 0
+    assert max(errs) <= TOL

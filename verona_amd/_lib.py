@@ -15,7 +15,7 @@ VRN_OK = 0
 REC = {"minmod": 0, "ppm": 1, "wenoz": 2}
 RIEMANN = {"hll": 0, "hllc": 1}
 INTEGRATOR = {"rk2": 0, "rk3": 1}
-PRECISION = {"fp64": 0, "fp32": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "fp64fast": 2}
 BC = {"none": 0, "periodic": 1, "outflow": 2, "nozzle": 3}
 SLOT = {"P": 0, "U": 1, "Uhalf": 2, "Fx": 3, "Fy": 4, "Fz": 5}
 

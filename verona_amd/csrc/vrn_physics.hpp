@@ -7,7 +7,19 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-namespace vrn {
+#include <type_traits>
+
+// VRN_NS: namespace of this instantiation.  With VRN_FAST the same functions
+// are built as the fast FP64 form: a*b+c contracted to FMA (the including
+// file sets the contraction pragma) and fdiv() = reciprocal refined by two
+// Newton steps times the numerator with one residual correction (within an
+// ulp of the correctly rounded quotient), used only where the divisor is
+// bounded away from zero and from the subnormal range.
+#ifndef VRN_NS
+#define VRN_NS vrn
+#endif
+
+namespace VRN_NS {
 
 // Julia Base max/min for IEEE floats: NaN-propagating, -0 < +0.
 template <class T> __host__ __device__ __forceinline__ T jmax(T x, T y)
@@ -34,6 +46,27 @@ __host__ __device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrt
 __host__ __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __host__ __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 
+#if defined(VRN_FAST)
+__host__ __device__ __forceinline__ double fdiv(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    double q = a * r;
+    double rem = __builtin_fma(-b, q, a);
+    return __builtin_fma(rem, r, q);
+#else
+    return a / b;
+#endif
+}
+#else
+__host__ __device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+#endif
+__host__ __device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
+
 enum { REC_MINMOD = 0, REC_PPM = 1, REC_WENOZ = 2 };
 enum { RIEM_HLL = 0, RIEM_HLLC = 1 };
 
@@ -56,20 +89,37 @@ __device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T 
     T ind0 = tau5 / (beta0 + eps);
     T ind1 = tau5 / (beta1 + eps);
     T ind2 = tau5 / (beta2 + eps);
-    T s0 = T(1) + sq(ind0), s1 = T(1) + sq(ind1), s2 = T(1) + sq(ind2);
+    T s0, s1, s2;
+    if constexpr (std::is_same<T, float>::value) {
+        // FP32 form: ind^2 overflows FP32 once ind > ~1.8e19 (a flat side next
+        // to a small jump gives tau5 / eps that large), and inf/inf would give
+        // NaN.  The weights enter only as ratios, so all three are scaled by a
+        // common factor 1/ind_max^2 when ind_max > 1e15.
+        T im = ind0 > ind1 ? ind0 : ind1;
+        im = im > ind2 ? im : ind2;
+        T sc = im > T(1e15) ? T(1) / im : T(1);
+        T sc2 = sc * sc;
+        s0 = sc2 + sq(ind0 * sc);
+        s1 = sc2 + sq(ind1 * sc);
+        s2 = sc2 + sq(ind2 * sc);
+    } else {
+        s0 = T(1) + sq(ind0);
+        s1 = T(1) + sq(ind1);
+        s2 = T(1) + sq(ind2);
+    }
 
     T a0 = T(0.1) * s0, a1 = T(0.6) * s1, a2 = T(0.3) * s2;
     T f0 = T(2) * qm2 - T(7) * qm1 + T(11) * q;
     T f1 = -qm1 + T(5) * q + T(2) * qp1;
     T f2 = T(2) * q + T(5) * qp1 - qp2;
-    hi = (f0 * a0 + f1 * a1 + f2 * a2) / (T(6) * (a0 + a1 + a2));
+    hi = fdiv(f0 * a0 + f1 * a1 + f2 * a2, T(6) * (a0 + a1 + a2));
 
     a0 = T(0.1) * s2;
     a2 = T(0.3) * s0;
     f0 = T(2) * qp2 - T(7) * qp1 + T(11) * q;
     f1 = -qp1 + T(5) * q + T(2) * qm1;
     f2 = T(2) * q + T(5) * qm1 - qm2;
-    lo = (f0 * a0 + f1 * a1 + f2 * a2) / (T(6) * (a0 + a1 + a2));
+    lo = fdiv(f0 * a0 + f1 * a1 + f2 * a2, T(6) * (a0 + a1 + a2));
 }
 
 // fluxlimiter.jl:1-13  MINMOD(q_{i-1}, q_i, q_{i+1})
@@ -215,7 +265,7 @@ __device__ __forceinline__ void p2f(const T *P, T *F, T g, T gam)
 // eos.jl:15-17
 template <class T> __host__ __device__ __forceinline__ T sound_speed(T rho, T u, T g)
 {
-    return dsqrt((g * (g - T(1)) * u) / (rho + g * u));
+    return dsqrt(fdiv(g * (g - T(1)) * u, rho + g * u));
 }
 
 // 3dim/algos.jl:146-157: sigma = c^2/(lor^2 (1-c^2)),
@@ -223,10 +273,16 @@ template <class T> __host__ __device__ __forceinline__ T sound_speed(T rho, T u,
 template <class T>
 __host__ __device__ __forceinline__ void lambdas(T v, T cs, T lor2, T &lp, T &lm)
 {
-    T sig = cs * cs / (lor2 * (T(1) - cs * cs));
+    T sig = fdiv(cs * cs, lor2 * (T(1) - cs * cs));
     T rt = dsqrt(sig * (T(1) - v * v + sig));
+#if defined(VRN_FAST)
+    T inv = fdiv(T(1), T(1) + sig);
+    lp = (v + rt) * inv;
+    lm = (v - rt) * inv;
+#else
     lp = (v + rt) / (T(1) + sig);
     lm = (v - rt) / (T(1) + sig);
+#endif
 }
 
 // HLL (3dim/algos.jl:158-174); HLLC with HLL fallback (Mignone & Bodo 2005,
@@ -247,7 +303,7 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
     }
     if constexpr (RIEM == RIEM_HLLC) {
         T SL = -Cmin, SR = Cmax;
-        T inv = T(1) / (Cmax + Cmin);
+        T inv = fdiv(T(1), Cmax + Cmin);
         T E_h = -(Cmax * UR[1] + Cmin * UL[1] + FL[1] - FR[1]) * inv;
         T m_h = (Cmax * UR[nd] + Cmin * UL[nd] + FL[nd] - FR[nd]) * inv;
         T FE_h = -(FR[1] * Cmin + FL[1] * Cmax - Cmax * Cmin * (UR[1] - UL[1])) * inv;
@@ -258,7 +314,7 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
         T lam = T(0), ps = T(0);
         if (ok) {
             T qq = b + dsqrt(disc);
-            lam = T(2) * m_h / qq;
+            lam = fdiv(T(2) * m_h, qq);
             ps = Fm_h - lam * FE_h;
             ok = __builtin_isfinite(lam) && lam > SL && lam < SR && ps > T(0);
         }
@@ -267,7 +323,7 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
             T S = left ? SL : SR;
             T vK = left ? vL : vR;
             T pK = left ? pL : pR;
-            T fac = T(1) / (S - lam);
+            T fac = fdiv(T(1), S - lam);
             T sv = S - vK;
             T Us[NV];
             T UK[NV], FK[NV];
@@ -289,9 +345,16 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
             }
         }
     }
+#if defined(VRN_FAST)
+    const T invs = fdiv(T(1), Cmax + Cmin);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        F[v] = (FR[v] * Cmin + FL[v] * Cmax - Cmax * Cmin * (UR[v] - UL[v])) * invs;
+#else
 #pragma unroll
     for (int v = 0; v < NV; v++)
         F[v] = (FR[v] * Cmin + FL[v] * Cmax - Cmax * Cmin * (UR[v] - UL[v])) / (Cmax + Cmin);
+#endif
 }
 
 // Face flux along DIM from floored face states PL (left) and PR (right):
@@ -305,8 +368,8 @@ __device__ __forceinline__ void face_flux(const T *PL, const T *PR, T g, T *F)
     p2u<NV>(PL, UL, g, lorL);
     p2f<NV, DIM>(PR, FR, g, lorR);
     p2f<NV, DIM>(PL, FL, g, lorL);
-    T vL = PL[2 + DIM] / lorL;
-    T vR = PR[2 + DIM] / lorR;
+    T vL = fdiv(PL[2 + DIM], lorL);
+    T vR = fdiv(PR[2 + DIM], lorR);
     T CL = sound_speed(PL[0], PL[1], g);
     T CR = sound_speed(PR[0], PR[1], g);
     T lpL, lmL, lpR, lmR;
@@ -329,7 +392,7 @@ __host__ __device__ __forceinline__ double cell_rate(const double *P, double g, 
 #pragma unroll
     for (int d = 0; d < NV - 2; d++) {
         double lp, lm;
-        lambdas(P[2 + d] / lor, cs, lor * lor, lp, lm);
+        lambdas(fdiv(P[2 + d], lor), cs, lor * lor, lp, lm);
         r = r + jmax(dabs(lp), dabs(lm)) * inv_dx[d];
     }
     return r;
@@ -354,7 +417,7 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
     for (int it = 0; it < n_iter; it++) {
         iters++;
         if (W < W_min) W = W_min;
-        double v2 = S2 / (W * W);
+        double v2 = fdiv(S2, W * W);
         if (v2 < 0.0)
             v2 = 0.0;
         else if (v2 > 1.0 - 1e-15)
@@ -362,8 +425,8 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
         double sv = dsqrt(1.0 - v2);
         double fun = Qt + W - c * (W * (1.0 - v2) - D * sv);
         double W3 = W * W * W;
-        double jac = 1.0 - c * (1.0 - v2) - c * (W * (2.0 * S2 / W3) - D * (S2 / (W3 * sv)));
-        double dW = fun / jac;
+        double jac = 1.0 - c * (1.0 - v2) - c * (W * fdiv(2.0 * S2, W3) - D * fdiv(S2, W3 * sv));
+        double dW = fdiv(fun, jac);
         double Wp = W - dW;
         W = (Wp < W_min) ? 0.5 * (W + W_min) : Wp;
         if (dW < 0.0) dW = -dW;
@@ -373,14 +436,21 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
         }
     }
     if (!conv) return false;
-    double v2 = S2 / (W * W);
-    double gam = 1.0 / dsqrt(1.0 - v2);
+    double v2 = fdiv(S2, W * W);
+    double gam = fdiv(1.0, dsqrt(1.0 - v2));
     gam = jmin(gam, 50.0);
-    P[0] = jmax(1e-8, D / gam);
-    P[1] = jmax(1e-8, (W / (gam * gam) - D / gam) / g);
+    P[0] = jmax(1e-8, fdiv(D, gam));
+    P[1] = jmax(1e-8, fdiv(fdiv(W, gam * gam) - fdiv(D, gam), g));
+#if defined(VRN_FAST)
+    const double iW = fdiv(gam, W);
+    P[2] = Qx * iW;
+    P[3] = Qy * iW;
+    P[4] = Qz * iW;
+#else
     P[2] = gam * Qx / W;
     P[3] = gam * Qy / W;
     P[4] = gam * Qz / W;
+#endif
     return true;
 }
 
@@ -689,4 +759,4 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
     return true;
 }
 
-}  // namespace vrn
+}  // namespace VRN_NS

@@ -1,0 +1,411 @@
+// vrn_sweeps.hpp -- the fused directional sweep kernels of vrn_step (see
+// vrn_grid.hip for the layout and the design).  Included once per arithmetic
+// form: VRN_NS names the namespace of the instantiation and VRN_FAST selects
+// the contracted / reciprocal-division FP64 form.
+#pragma once
+#include "vrn_types.hpp"
+#include "vrn_physics.hpp"
+
+#include <type_traits>
+
+namespace VRN_NS {
+using vrn::Geo;
+using vrn::DevState;
+using vrn::SweepArgs;
+using vrn::gidx;
+using vrn::XOFF;
+
+// ------------------------------------------------------------ wave helpers
+template <class T> __device__ __forceinline__ T shift_up1(T v) { return __shfl_up(v, 1); }
+template <class T> __device__ __forceinline__ T lane63(T v) { return __shfl(v, 63); }
+
+__device__ __forceinline__ double wave_max(double r)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off));
+    return r;
+}
+
+__device__ __forceinline__ void rate_reduce(DevState *st, double r)
+{
+    if (!(r >= 0.0)) r = 0.0;  // NaN rates are ignored, as in a `r > max` scan
+    r = wave_max(r);
+    if (__lane_id() == 0) atomicMax(&st->rate_bits, (unsigned long long)__double_as_longlong(r));
+}
+
+__device__ __forceinline__ void rescue_push(int *list, DevState *st, bool need, int idx)
+{
+    unsigned long long m = __ballot(need);
+    if (m == 0) return;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0;
+    if (__lane_id() == leader) base = atomicAdd(&st->rescue_count, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    if (need) {
+        unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
+        list[base + __popcll(below)] = idx;
+    }
+}
+
+template <int NV> __device__ __forceinline__ bool all_finite(const double *p)
+{
+    bool ok = true;
+#pragma unroll
+    for (int v = 0; v < NV; v++) ok = ok && __builtin_isfinite(p[v]);
+    return ok;
+}
+
+// Combine, con2prim, Limit, rate for one updated cell of the last sweep.
+// A cell counts as unrecovered when its con2prim result is not finite (the
+// reference's NaN marker, or a NaN from a branch that reported convergence).
+template <int NV>
+__device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const double *acc,
+                                            const double *u0, const double *guess, bool valid,
+                                            double &rate)
+{
+    const long ps = a.geo.ps;
+    double out[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = (a.a == 0.0) ? acc[v] : a.a * u0[v] + a.b * acc[v];
+    bool need = false;
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) a.Uout[v * ps + cu] = out[v];
+        double Pn[NV];
+        int iters = 0;
+        bool ok;
+        if constexpr (NV == 5) {
+            ok = c2p_1dw(out, guess, a.g, a.n_iter, a.tol, Pn, iters);
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) Pn[v] = guess[v];
+            ok = c2p_2d(out, Pn, a.g, a.n_iter, a.tol, iters);
+            if (!ok) atomicAdd(&a.st->n_singular, 1ull);
+            ok = true;
+        }
+        if (ok) {
+            if (!all_finite<NV>(Pn)) atomicAdd(&a.st->n_unrecovered, 1ull);
+            Pn[0] = jmax(Pn[0], a.floor_);
+            Pn[1] = jmax(Pn[1], a.floor_);
+#pragma unroll
+            for (int v = 0; v < NV; v++) a.Pout[v * ps + cu] = Pn[v];
+            if (a.do_rate) rate = fmax(rate, cell_rate<NV>(Pn, a.g, a.inv_dx));
+        } else {
+            need = true;
+        }
+    }
+    if constexpr (NV == 5) rescue_push(a.rescue, a.st, need, (int)cu);
+}
+
+// ------------------------------------------------------------ x sweep
+// Lanes walk a stream of (row, cell) pairs: rows are interior (j, k), cells
+// c = 2 .. NX-3 hold the reconstructions; lane with cell c also solves face
+// c-1/2 (c >= 3) and updates cell c-1 (c >= 4).
+template <int NV, class R, int REC, int RIEM, bool LAST>
+__global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
+{
+    const Geo G = a.geo;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int Nyi = G.NY - 6;
+    const int Nzi = NV == 5 ? G.NZ - 6 : 1;
+    const int nrows = Nyi * Nzi;
+    const int r0 = wave * a.rows_per_wave;
+    if (r0 >= nrows) return;
+    const int r1 = min(r0 + a.rows_per_wave, nrows);
+    const int E = G.NX - 4;
+    const int total = (r1 - r0) * E;
+    const long ps = G.ps;
+    const R g = (R)a.g, fl = (R)a.floor_;
+    const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
+    const double *__restrict__ P = a.P;
+
+    R cup[NV];
+    double cF[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        cup[v] = R(0);
+        cF[v] = 0.0;
+    }
+    double rate = 0.0;
+    for (int s0 = 0; s0 < total; s0 += 64) {
+        const int s = s0 + lane;
+        const bool act = s < total;
+        const int ss = act ? s : total - 1;
+        const int r = r0 + ss / E;
+        const int c = 2 + ss % E;
+        const int j = 3 + r % Nyi;
+        const int k = NV == 5 ? 3 + r / Nyi : 0;
+        const long rb = gidx(G, 0, j, k);
+        R lo[NV], up[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double *q = P + v * ps + rb + c;
+            recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
+        }
+        lo[0] = jmax(fl, lo[0]);
+        lo[1] = jmax(fl, lo[1]);
+        up[0] = jmax(fl, up[0]);
+        up[1] = jmax(fl, up[1]);
+        R upl[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            R t = shift_up1(up[v]);
+            upl[v] = lane == 0 ? cup[v] : t;
+        }
+        R Ff[NV];
+        face_flux<NV, 0, RIEM>(upl, lo, g, Ff);
+        double F[NV], Fl[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            F[v] = (double)Ff[v];
+            double t = shift_up1(F[v]);
+            Fl[v] = lane == 0 ? cF[v] : t;
+        }
+        const bool upd = act && c >= 4;
+        const long cu = rb + (c - 1);
+        if constexpr (!LAST) {
+            if (upd) {
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    a.Uacc[v * ps + cu] = a.Ub[v * ps + cu] - cdt * (F[v] - Fl[v]);
+            }
+        } else {
+            // 2D: the x sweep is never last; kept for completeness of the template.
+            double acc[NV], u0[NV], guess[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                acc[v] = upd ? a.Ub[v * ps + cu] - cdt * (F[v] - Fl[v]) : 0.0;
+                u0[v] = (upd && a.a != 0.0) ? a.U0[v * ps + cu] : 0.0;
+                guess[v] = upd ? P[v * ps + cu] : 1.0;
+            }
+            finish_cell<NV>(a, cu, acc, u0, guess, upd, rate);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            cup[v] = lane63(up[v]);
+            cF[v] = lane63(F[v]);
+        }
+    }
+    if constexpr (LAST)
+        if (a.do_rate) rate_reduce(a.st, rate);
+}
+
+// ------------------------------------------------------------ y / z sweeps
+// One lane per interior (x, o) column, marching along DIM over the segment
+// [m0, m1) of interior cells with the 5-cell window in registers.
+template <int NV, int DIM, class R, int REC, int RIEM, bool LAST>
+__global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
+{
+    const Geo G = a.geo;
+    const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int o = 3 + blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int Nd = DIM == 1 ? G.NY : G.NZ;
+    const int No = NV == 5 ? (DIM == 1 ? G.NZ : G.NY) : 7;
+    const int m0 = 3 + blockIdx.z * a.seg_len;
+    // whole-wave exits only: a partially active wave keeps all lanes for the
+    // cross-lane reductions and masks the inactive ones.
+    if (o >= No - 3 || m0 >= Nd - 3) return;
+    const bool act = x < G.NX - 3;
+    const int xx = act ? x : 3;
+    const int m1 = min(m0 + a.seg_len, Nd - 3);
+    const long sd = DIM == 1 ? (long)G.px : (long)G.px * G.NY;
+    const long cb = DIM == 1 ? gidx(G, xx, 0, NV == 5 ? o : 0) : gidx(G, xx, o, 0);
+    const long ps = G.ps;
+    const R g = (R)a.g, fl = (R)a.floor_;
+    const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
+    const double *__restrict__ P = a.P;
+    const double *Ui = a.Uacc;
+
+    R w0[NV], w1[NV], w2[NV], w3[NV], w4[NV];
+    double nx[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double *q = P + v * ps + cb;
+        w0[v] = (R)q[(m0 - 3) * sd];
+        w1[v] = (R)q[(m0 - 2) * sd];
+        w2[v] = (R)q[(m0 - 1) * sd];
+        w3[v] = (R)q[(m0)*sd];
+        nx[v] = q[(m0 + 1) * sd];
+    }
+    R upp[NV];
+    double Fp[NV], ub[NV], u0b[NV], gb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        upp[v] = R(0);
+        Fp[v] = 0.0;
+        ub[v] = 0.0;
+        u0b[v] = 0.0;
+        gb[v] = 0.0;
+    }
+    double rate = 0.0;
+    for (int c = m0 - 1; c <= m1; c++) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) w4[v] = (R)nx[v];
+        if (c + 3 < Nd) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) nx[v] = P[v * ps + cb + (c + 3) * sd];
+        }
+        // prefetch the conserved state of cell c, updated at iteration c+1
+        double un[NV], u0n[NV], gn[NV];
+        const bool pre = c >= m0 && c < m1;
+        const long cc = cb + (long)c * sd;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            un[v] = pre ? Ui[v * ps + cc] : 0.0;
+            u0n[v] = (LAST && pre && a.a != 0.0) ? a.U0[v * ps + cc] : 0.0;
+            if constexpr (LAST && !std::is_same<R, double>::value) gn[v] = pre ? P[v * ps + cc] : 0.0;
+        }
+        R lo[NV], up[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) recon<REC>(w0[v], w1[v], w2[v], w3[v], w4[v], lo[v], up[v]);
+        lo[0] = jmax(fl, lo[0]);
+        lo[1] = jmax(fl, lo[1]);
+        up[0] = jmax(fl, up[0]);
+        up[1] = jmax(fl, up[1]);
+        if (c >= m0) {
+            R Ff[NV];
+            face_flux<NV, DIM, RIEM>(upp, lo, g, Ff);
+            double F[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) F[v] = (double)Ff[v];
+            if (c - 1 >= m0) {
+                const long cu = cb + (long)(c - 1) * sd;
+                double acc[NV];
+#pragma unroll
+                for (int v = 0; v < NV; v++) acc[v] = ub[v] - cdt * (F[v] - Fp[v]);
+                if constexpr (!LAST) {
+                    if (act) {
+#pragma unroll
+                        for (int v = 0; v < NV; v++) a.Uacc[v * ps + cu] = acc[v];
+                    }
+                } else {
+                    double guess[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        if constexpr (std::is_same<R, double>::value)
+                            guess[v] = w1[v];
+                        else
+                            guess[v] = gb[v];
+                    }
+                    finish_cell<NV>(a, cu, acc, u0b, guess, act, rate);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NV; v++) Fp[v] = F[v];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            upp[v] = up[v];
+            ub[v] = un[v];
+            u0b[v] = u0n[v];
+            if constexpr (LAST && !std::is_same<R, double>::value) gb[v] = gn[v];
+            w0[v] = w1[v];
+            w1[v] = w2[v];
+            w2[v] = w3[v];
+            w3[v] = w4[v];
+        }
+    }
+    if constexpr (LAST)
+        if (a.do_rate) rate_reduce(a.st, rate);
+}
+
+// ------------------------------------------------------------ rescue (3D)
+// Cells whose 1DW Newton did not converge: rest of the cascade from the
+// stage's guess, then Limit.  The list holds linear device indices.
+__global__ __launch_bounds__(64) void k_rescue(SweepArgs a)
+{
+    const unsigned int n = a.st->rescue_count;
+    const long ps = a.geo.ps;
+    double rate = 0.0;
+    for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const long cu = a.rescue[t];
+        double U[5], Pn[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            U[v] = a.Uout[v * ps + cu];
+            Pn[v] = a.P[v * ps + cu];
+        }
+        int iters = 0;
+        c2p_cascade_tail(U, Pn, a.g, a.n_iter, a.tol, iters);
+        if (!all_finite<5>(Pn)) atomicAdd(&a.st->n_unrecovered, 1ull);
+        Pn[0] = jmax(Pn[0], a.floor_);
+        Pn[1] = jmax(Pn[1], a.floor_);
+#pragma unroll
+        for (int v = 0; v < 5; v++) a.Pout[v * ps + cu] = Pn[v];
+        if (a.do_rate) rate = fmax(rate, cell_rate<5>(Pn, a.g, a.inv_dx));
+    }
+    if (a.do_rate) rate_reduce(a.st, rate);
+}
+
+
+// Launch the three sweeps + rescue of one RK stage.
+template <int NV, class R, int REC, int RIEM>
+void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (*timer)(void *, int, int), void *tctx)
+{
+    const Geo &G = a.geo;
+    {
+        SweepArgs ax = a;
+        ax.dx_dim = dxs[0];
+        int nrows = (G.NY - 6) * (NV == 5 ? G.NZ - 6 : 1);
+        int waves = (nrows + ax.rows_per_wave - 1) / ax.rows_per_wave;
+        timer(tctx, 0, 0);
+        hipLaunchKernelGGL((k_xsweep<NV, R, REC, RIEM, false>), dim3((waves + 3) / 4), dim3(256), 0, s, ax);
+        timer(tctx, 0, 1);
+    }
+    const int nxb = (G.NX - 6 + 63) / 64;
+    if constexpr (NV == 5) {
+        {
+            SweepArgs ay = a;
+            ay.dx_dim = dxs[1];
+            int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
+            timer(tctx, 1, 0);
+            hipLaunchKernelGGL((k_march<5, 1, R, REC, RIEM, false>), dim3(nxb, (G.NZ - 6 + 3) / 4, nseg), dim3(256), 0, s, ay);
+            timer(tctx, 1, 1);
+        }
+        {
+            SweepArgs az = a;
+            az.dx_dim = dxs[2];
+            int nseg = (G.NZ - 6 + a.seg_len - 1) / a.seg_len;
+            timer(tctx, 2, 0);
+            hipLaunchKernelGGL((k_march<5, 2, R, REC, RIEM, true>), dim3(nxb, (G.NY - 6 + 3) / 4, nseg), dim3(256), 0, s, az);
+            timer(tctx, 2, 1);
+        }
+        {
+            timer(tctx, 4, 0);
+            hipLaunchKernelGGL(k_rescue, dim3(256), dim3(64), 0, s, a);
+            timer(tctx, 4, 1);
+        }
+    } else {
+        SweepArgs ay = a;
+        ay.dx_dim = dxs[1];
+        int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
+        timer(tctx, 3, 0);
+        hipLaunchKernelGGL((k_march<4, 1, R, REC, RIEM, true>), dim3(nxb, 1, nseg), dim3(64), 0, s, ay);
+        timer(tctx, 3, 1);
+    }
+}
+
+template <int NV>
+void dispatch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], int rec, int riem, bool fp32,
+                    void (*timer)(void *, int, int), void *tctx)
+{
+#define VRN_ST(R_, REC_, RI_) launch_stage<NV, R_, REC_, RI_>(s, a, dxs, timer, tctx)
+#define VRN_REC(R_, RI_)                                        \
+    do {                                                        \
+        if (rec == REC_WENOZ) VRN_ST(R_, REC_WENOZ, RI_);       \
+        else if (rec == REC_PPM) VRN_ST(R_, REC_PPM, RI_);      \
+        else VRN_ST(R_, REC_MINMOD, RI_);                       \
+    } while (0)
+    if (!fp32) {
+        if (riem == RIEM_HLL) VRN_REC(double, RIEM_HLL);
+        else VRN_REC(double, RIEM_HLLC);
+    } else {
+        if (riem == RIEM_HLL) VRN_REC(float, RIEM_HLL);
+        else VRN_REC(float, RIEM_HLLC);
+    }
+#undef VRN_REC
+#undef VRN_ST
+}
+
+}  // namespace VRN_NS

@@ -1,0 +1,59 @@
+// vrn_types.hpp -- device layout and argument blocks shared by the grid kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#ifndef VRN_MARCH_MIN_WAVES
+#define VRN_MARCH_MIN_WAVES 2  // measured: 2 waves/SIMD beats the unbounded allocation
+#endif
+
+namespace vrn {
+
+constexpr int XOFF = 13;
+
+struct Geo {
+    int NX, NY, NZ;  // with ghosts; NZ = 1 in 2D
+    int px;          // row pitch in doubles
+    long ps;         // plane stride in doubles
+};
+
+__host__ __device__ __forceinline__ long gidx(const Geo &G, int i, int j, int k)
+{
+    return XOFF + i + (long)G.px * (j + (long)G.NY * k);
+}
+
+struct DevState {
+    double dt;       // dt of the current step (CFL mode)
+    double dt_prev;  // dt of the step that just finished
+    unsigned long long rate_bits;   // max signal rate (non-negative double bits)
+    unsigned int rescue_count;
+    unsigned int pad0;
+    unsigned long long n_unrecovered;
+    unsigned long long n_rescued;
+    unsigned long long n_singular;
+};
+
+struct SweepArgs {
+    Geo geo;
+    const double *P;   // primitives of this stage (read)
+    const double *U0;  // U^n (last sweep, a != 0)
+    const double *Ub;  // base state (first sweep)
+    double *Uacc;      // accumulator (first: write; middle: read/write; last: read)
+    double *Uout;      // last sweep: combined conserved state
+    double *Pout;      // last sweep: recovered primitives
+    DevState *st;
+    int *rescue;
+    double g, floor_;
+    double inv_dx[3];
+    double dtfac;     // stage dt = dt * dtfac
+    double dx_dim;    // spacing along this sweep's axis
+    double dt_fixed;  // dt when dt_dev == 0
+    int dt_dev;       // 1: dt comes from DevState (CFL mode)
+    double a, b;    // stage output = a U0 + b acc (a == 0: acc)
+    double tol;
+    int n_iter;
+    int do_rate;
+    int seg_len;
+    int rows_per_wave;
+};
+
+}  // namespace vrn
