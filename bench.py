@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--n", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device copies of one 384^3 slot (known bytes) for counter calibration")
     ap.add_argument("--precision", default="fp64fast", choices=["fp64", "fp64fast"],
                     help="FP64 form of the headline run (both agree with the oracle to <= 1e-12)")
     args = ap.parse_args()
@@ -142,6 +144,15 @@ def main():
               "(see DESIGN.md for how the grid would shard).", file=sys.stderr)
         sys.exit(2)
     n, K, W = args.n, args.steps, args.warmup
+    if args.copy_calibration:
+        from verona_amd import DeviceGrid
+        with DeviceGrid(3, n, n, n, GAMMA) as dev:
+            for _ in range(3):
+                dev.copy("Fx", "P")
+            dev.sync()
+            print(json.dumps({"copy_bytes_read": dev.slot_bytes(), "copy_bytes_written": dev.slot_bytes(),
+                              "copies": 3}), flush=True)
+        return
     cells = n ** 3
     el, rep, nbad, nresc = run_gpu(n, K, W, ref=False, precision=args.precision)
     value = cells * K / el

@@ -91,6 +91,12 @@ const char *vrn_last_error(void);
 int vrn_upload(vrn_ctx *ctx, int slot, const double *host);
 int vrn_download(vrn_ctx *ctx, int slot, double *host);
 
+/* Device-to-device copy of a whole slot (no reference counterpart: a
+ * snapshot utility, also the known-size copy used to calibrate HBM byte
+ * counters); vrn_slot_bytes gives the bytes one slot occupies on the device. */
+int vrn_copy(vrn_ctx *ctx, int dst_slot, int src_slot);
+int vrn_slot_bytes(vrn_ctx *ctx, int64_t *bytes);
+
 /* PtoU(P, U, eos): U = conserved(P) over the whole array,
  * src/3dim/structs.jl:66-81 (2D: src/2dim/structs.jl:62-76). */
 int vrn_prim_to_cons(vrn_ctx *ctx);

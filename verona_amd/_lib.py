@@ -50,6 +50,8 @@ SIGNATURES = {
     "vrn_upload": (_I, [_P, _I, _DP]),
     "vrn_download": (_I, [_P, _I, _DP]),
     "vrn_prim_to_cons": (_I, [_P]),
+    "vrn_copy": (_I, [_P, _I, _I]),
+    "vrn_slot_bytes": (_I, [_P, _I64P]),
     "vrn_fluxes": (_I, [_P, _I, _D, _I, _I]),
     "vrn_update": (_I, [_P, _I, _D, _D, _D, _D]),
     "vrn_cons_to_prim": (_I, [_P, _I, _I, _D]),

@@ -667,6 +667,22 @@ int vrn_download(vrn_ctx *c, int slot, double *host)
     return 0;
 }
 
+int vrn_copy(vrn_ctx *c, int dst_slot, int src_slot)
+{
+    if (int r = check_ctx(c)) return r;
+    double *d = slot_ptr(c, dst_slot), *s_ = slot_ptr(c, src_slot);
+    if (!d || !s_ || d == s_) return fail(VRN_E_ARG, "bad slots %d <- %d", dst_slot, src_slot);
+    HIPCHK(hipMemcpyAsync(d, s_, sizeof(double) * (size_t)c->NV * c->G.ps, hipMemcpyDeviceToDevice, c->s));
+    return 0;
+}
+
+int vrn_slot_bytes(vrn_ctx *c, int64_t *bytes)
+{
+    if (int r = check_ctx(c)) return r;
+    if (bytes) *bytes = (int64_t)(sizeof(double) * (size_t)c->NV * c->G.ps);
+    return 0;
+}
+
 int vrn_prim_to_cons(vrn_ctx *c)
 {
     if (int r = check_ctx(c)) return r;

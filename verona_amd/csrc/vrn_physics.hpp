@@ -86,9 +86,10 @@ __device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T 
     T beta2 = b1c * sq(qp2 + q - T(2) * qp1) + b2c * sq(qp2 + T(3) * q - T(4) * qp1);
     const T eps = WenoEps<T>::v;
     T tau5 = dabs(beta0 - beta2);
-    T ind0 = tau5 / (beta0 + eps);
-    T ind1 = tau5 / (beta1 + eps);
-    T ind2 = tau5 / (beta2 + eps);
+    // beta + eps >= 1e-42, a normal double, so the fast reciprocal is safe here
+    T ind0 = fdiv(tau5, beta0 + eps);
+    T ind1 = fdiv(tau5, beta1 + eps);
+    T ind2 = fdiv(tau5, beta2 + eps);
     T s0, s1, s2;
     if constexpr (std::is_same<T, float>::value) {
         // FP32 form: ind^2 overflows FP32 once ind > ~1.8e19 (a flat side next

@@ -71,6 +71,14 @@ class DeviceGrid:
     def sync(self):
         check(self.lib.vrn_sync(self.h), "vrn_sync")
 
+    def copy(self, dst, src):
+        check(self.lib.vrn_copy(self.h, SLOT[dst], SLOT[src]), "vrn_copy")
+
+    def slot_bytes(self):
+        b = C.c_int64()
+        check(self.lib.vrn_slot_bytes(self.h, C.byref(b)), "vrn_slot_bytes")
+        return b.value
+
     # ------------------------------------------------------------------ reference kernel objects
     def prim_to_cons(self):
         check(self.lib.vrn_prim_to_cons(self.h), "vrn_prim_to_cons")
