@@ -16,8 +16,35 @@ using vrn::gidx;
 using vrn::XOFF;
 
 // ------------------------------------------------------------ wave helpers
-template <class T> __device__ __forceinline__ T shift_up1(T v) { return __shfl_up(v, 1); }
-template <class T> __device__ __forceinline__ T lane63(T v) { return __shfl(v, 63); }
+// Value of lane l-1 (lane 0 gets garbage and is overridden by the caller):
+// DPP wave_shr:1 (GFX9 wave-wide shift), a VALU modifier -- no LDS round trip.
+__device__ __forceinline__ int dpp_shr1(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float shift_up1(float v)
+{
+    return __int_as_float(dpp_shr1(__float_as_int(v)));
+}
+__device__ __forceinline__ double shift_up1(double v)
+{
+    long long b = __double_as_longlong(v);
+    int lo = dpp_shr1((int)(b & 0xffffffffll));
+    int hi = dpp_shr1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Value of lane 63 in every lane (v_readlane into a scalar register).
+__device__ __forceinline__ float lane63(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 __device__ __forceinline__ double wave_max(double r)
 {
@@ -137,6 +164,12 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
         const int j = 3 + r % Nyi;
         const int k = NV == 5 ? 3 + r / Nyi : 0;
         const long rb = gidx(G, 0, j, k);
+        const bool upd = act && c >= 4;
+        const long cu = rb + (c - 1);
+        // base state of the cell this lane updates, issued before the flux work
+        double ub[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) ub[v] = upd ? a.Ub[v * ps + cu] : 0.0;
         R lo[NV], up[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -162,20 +195,17 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
             double t = shift_up1(F[v]);
             Fl[v] = lane == 0 ? cF[v] : t;
         }
-        const bool upd = act && c >= 4;
-        const long cu = rb + (c - 1);
         if constexpr (!LAST) {
             if (upd) {
 #pragma unroll
-                for (int v = 0; v < NV; v++)
-                    a.Uacc[v * ps + cu] = a.Ub[v * ps + cu] - cdt * (F[v] - Fl[v]);
+                for (int v = 0; v < NV; v++) a.Uacc[v * ps + cu] = ub[v] - cdt * (F[v] - Fl[v]);
             }
         } else {
             // 2D: the x sweep is never last; kept for completeness of the template.
             double acc[NV], u0[NV], guess[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                acc[v] = upd ? a.Ub[v * ps + cu] - cdt * (F[v] - Fl[v]) : 0.0;
+                acc[v] = upd ? ub[v] - cdt * (F[v] - Fl[v]) : 0.0;
                 u0[v] = (upd && a.a != 0.0) ? a.U0[v * ps + cu] : 0.0;
                 guess[v] = upd ? P[v * ps + cu] : 1.0;
             }
@@ -193,11 +223,16 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
 
 // ------------------------------------------------------------ y / z sweeps
 // One lane per interior (x, o) column, marching along DIM over the segment
-// [m0, m1) of interior cells with the 5-cell window in registers.
+// [m0, m1) of interior cells.  The 5-cell reconstruction window of each lane
+// lives in LDS as a ring of 5 slots laid out [slot][var][lane] (each lane
+// touches only its own words: conflict-free and barrier-free), which keeps
+// it out of the VGPR budget of the Riemann solve and con2prim.
 template <int NV, int DIM, class R, int REC, int RIEM, bool LAST>
 __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 {
+    __shared__ R win[5 * NV * 256];
     const Geo G = a.geo;
+    const int tid = threadIdx.x;
     const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
     const int o = 3 + blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int Nd = DIM == 1 ? G.NY : G.NZ;
@@ -216,49 +251,50 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
     const double *__restrict__ P = a.P;
     const double *Ui = a.Uacc;
+    R *wl = win + tid;  // slot s, variable v at wl[(s * NV + v) * 256]
 
-    R w0[NV], w1[NV], w2[NV], w3[NV], w4[NV];
+    // ring: cell m lives in slot (m - (m0 - 3)) mod 5
     double nx[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         const double *q = P + v * ps + cb;
-        w0[v] = (R)q[(m0 - 3) * sd];
-        w1[v] = (R)q[(m0 - 2) * sd];
-        w2[v] = (R)q[(m0 - 1) * sd];
-        w3[v] = (R)q[(m0)*sd];
+#pragma unroll
+        for (int k = 0; k < 4; k++) wl[(k * NV + v) * 256] = (R)q[(m0 - 3 + k) * sd];
         nx[v] = q[(m0 + 1) * sd];
     }
     R upp[NV];
-    double Fp[NV], ub[NV], u0b[NV], gb[NV];
+    double Fp[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         upp[v] = R(0);
         Fp[v] = 0.0;
-        ub[v] = 0.0;
-        u0b[v] = 0.0;
-        gb[v] = 0.0;
     }
     double rate = 0.0;
+    int s0 = 0;  // slot of cell c - 2
     for (int c = m0 - 1; c <= m1; c++) {
+        const int s1 = s0 == 4 ? 0 : s0 + 1, s2 = s1 == 4 ? 0 : s1 + 1;
+        const int s3 = s2 == 4 ? 0 : s2 + 1, s4 = s3 == 4 ? 0 : s3 + 1;
 #pragma unroll
-        for (int v = 0; v < NV; v++) w4[v] = (R)nx[v];
+        for (int v = 0; v < NV; v++) wl[(s4 * NV + v) * 256] = (R)nx[v];
         if (c + 3 < Nd) {
 #pragma unroll
             for (int v = 0; v < NV; v++) nx[v] = P[v * ps + cb + (c + 3) * sd];
         }
-        // prefetch the conserved state of cell c, updated at iteration c+1
-        double un[NV], u0n[NV], gn[NV];
-        const bool pre = c >= m0 && c < m1;
-        const long cc = cb + (long)c * sd;
+        // conserved state of cell c-1 (updated at the end of this iteration):
+        // issued now, consumed after the reconstruction and the Riemann solve
+        const bool upd = c - 1 >= m0;
+        const long cu = cb + (long)(c - 1) * sd;
+        double ub[NV], u0b[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            un[v] = pre ? Ui[v * ps + cc] : 0.0;
-            u0n[v] = (LAST && pre && a.a != 0.0) ? a.U0[v * ps + cc] : 0.0;
-            if constexpr (LAST && !std::is_same<R, double>::value) gn[v] = pre ? P[v * ps + cc] : 0.0;
+            ub[v] = upd ? Ui[v * ps + cu] : 0.0;
+            u0b[v] = (LAST && upd && a.a != 0.0) ? a.U0[v * ps + cu] : 0.0;
         }
         R lo[NV], up[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) recon<REC>(w0[v], w1[v], w2[v], w3[v], w4[v], lo[v], up[v]);
+        for (int v = 0; v < NV; v++)
+            recon<REC>(wl[(s0 * NV + v) * 256], wl[(s1 * NV + v) * 256], wl[(s2 * NV + v) * 256],
+                       wl[(s3 * NV + v) * 256], wl[(s4 * NV + v) * 256], lo[v], up[v]);
         lo[0] = jmax(fl, lo[0]);
         lo[1] = jmax(fl, lo[1]);
         up[0] = jmax(fl, up[0]);
@@ -269,8 +305,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
             double F[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) F[v] = (double)Ff[v];
-            if (c - 1 >= m0) {
-                const long cu = cb + (long)(c - 1) * sd;
+            if (upd) {
                 double acc[NV];
 #pragma unroll
                 for (int v = 0; v < NV; v++) acc[v] = ub[v] - cdt * (F[v] - Fp[v]);
@@ -280,13 +315,14 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                         for (int v = 0; v < NV; v++) a.Uacc[v * ps + cu] = acc[v];
                     }
                 } else {
+                    // warm-start guess: the stage's primitives of cell c-1
                     double guess[NV];
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
                         if constexpr (std::is_same<R, double>::value)
-                            guess[v] = w1[v];
+                            guess[v] = wl[(s1 * NV + v) * 256];
                         else
-                            guess[v] = gb[v];
+                            guess[v] = P[v * ps + cu];
                     }
                     finish_cell<NV>(a, cu, acc, u0b, guess, act, rate);
                 }
@@ -295,16 +331,8 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
             for (int v = 0; v < NV; v++) Fp[v] = F[v];
         }
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            upp[v] = up[v];
-            ub[v] = un[v];
-            u0b[v] = u0n[v];
-            if constexpr (LAST && !std::is_same<R, double>::value) gb[v] = gn[v];
-            w0[v] = w1[v];
-            w1[v] = w2[v];
-            w2[v] = w3[v];
-            w3[v] = w4[v];
-        }
+        for (int v = 0; v < NV; v++) upp[v] = up[v];
+        s0 = s1;
     }
     if constexpr (LAST)
         if (a.do_rate) rate_reduce(a.st, rate);
