@@ -126,6 +126,18 @@ def cpu_baseline(budget_s=15.0, n=128):
                       f"workload at {n}^3, {steps} SSP-RK3 WENOZ+HLLC steps, {el:.1f} s"}
 
 
+def measured_traffic(kernel, n, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes (copy-calibrated), when they were taken on this workload and form; else None."""
+    path = os.path.join(ROOT, "profiles", "jet384_rk3_hbm_traffic.json")
+    if n != 384 or precision != "fp64fast" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else float(k["bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,9 +195,11 @@ def main():
                    "grid": [n, n, n], "gamma": GAMMA, "nozzle_radius_cells": 8,
                    "lorentz": 10.0, "bc": "nozzle z-low, outflow elsewhere"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, n, args.precision),
+                     "traffic_source": "profiles/jet384_rk3_hbm_traffic.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)",
                      "bytes_per_cell_per_launch": per_cell, "mean_launch_ms": mean_ms,
-                     "launches": cnt},
+                     "launches": cnt, "algorithmic_bytes": per_cell * cells},
         "roofline_step": {"bytes_per_zone_update": BYTES_PER_ZU["rk3"], "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "kernels_ms": {k: {"launches": c, "mean_ms": m} for k, (c, m) in rep.items()},
