@@ -126,6 +126,65 @@ def cpu_baseline(budget_s=15.0, n=128):
                       f"workload at {n}^3, {steps} SSP-RK3 WENOZ+HLLC steps, {el:.1f} s"}
 
 
+def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None):
+    """Learned-guess con2prim vs plain Newton on held-out cells of the benchmark
+    law (seed 42): every guess finished by the same FP64 1DW Newton to
+    |dW| < 1e-12 W.  Training pairs: the solver's own dumps (2D Kelvin-Helmholtz,
+    1D blast wave) plus synthetic cells drawn with seed 7."""
+    from verona_amd.c2p import Con2PrimBatch, c4_states, solver_pairs, training_set, warm_guess
+    say = log or (lambda m: None)
+    say("c2p: solver dumps")
+    dumps = solver_pairs()
+    Ut, Pt = training_set(n_train, seed=7, gamma=GAMMA, dumps=dumps)
+    say("c2p: held-out cells")
+    U, P = c4_states(n_cells, seed=42, gamma=GAMMA)
+    Pprev = warm_guess(P)
+    out = {"n_cells": n_cells, "rtol": 1e-12, "n_iter": 100, "gamma": GAMMA,
+           "held_out": "rho ~ logU[1e-4,1e2], p ~ logU[1e-6,1e2], Gamma ~ logU[1,100], isotropic, seed 42",
+           "warm": "previous primitives = true P x (1 + 0.01 N(0,1)) per component, seed 43",
+           "generic": "P = (1, 1, 0, 0, 0)", "analytic": "W0 = E = -T^t_t"}
+    with Con2PrimBatch(n_cells, GAMMA) as cp:
+        say("c2p: training")
+        curve, tms = cp.train(Ut, Pt, steps=train_steps, batch=8192, lr=3e-3, seed=7)
+        out["train"] = {"pairs": int(len(Ut)), "solver_dump_pairs": int(min(len(dumps[0]), n_train // 4)),
+                        "steps": train_steps, "batch": 8192, "ms": tms,
+                        "loss_first": float(curve[:10].mean()), "loss_last": float(curve[-100:].mean())}
+        cp.upload(U, Pprev)
+        res = {}
+        for g in ("generic", "warm", "analytic", "learned"):
+            say(f"c2p: solve {g}")
+            cp.solve(g)
+            runs = [cp.solve(g) for _ in range(reps)]
+            st = min(runs, key=lambda r: r["ms"])
+            st.pop("hist")
+            st["ms_median"] = float(np.median([r["ms"] for r in runs]))
+            st["cells_per_s"] = n_cells / (st["ms_median"] * 1e-3)
+            res[g] = st
+        Pl = cp.download()
+        cp.solve("analytic")
+        Pa = cp.download()
+        relL1 = (np.abs(Pl - Pa).sum(0) / np.abs(Pa).sum(0)).tolist()
+        # cells above the reference's Gamma <= 50 cap, on their own
+        lor = np.sqrt(1.0 + (P[:, 2:] ** 2).sum(1))
+        hi = lor > 50.0
+        cp.upload(U[hi], Pprev[hi])
+        sub = {}
+        for g in ("generic", "warm", "analytic", "learned"):
+            r = cp.solve(g)
+            sub[g] = {k: r[k] for k in ("n", "n_unrecovered", "n_fallback", "iters_mean", "iters_p99",
+                                        "iters_max", "wave_max_mean")}
+    for g in res:
+        res[g]["speedup_learned"] = res["learned"]["cells_per_s"] / res[g]["cells_per_s"]
+    out["guesses"] = res
+    out["learned_vs_analytic_rel_l1"] = relL1
+    out["gamma_above_50"] = sub
+    bytes_cell = 80
+    ach = bytes_cell * n_cells / (res["learned"]["ms_median"] * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "kernel": "k_solve<learned>", "bytes_per_cell": bytes_cell,
+                       "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+    return out
+
+
 def measured_traffic(kernel, n, precision):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     passes (copy-calibrated), when they were taken on this workload and form; else None."""
@@ -146,6 +205,8 @@ def main():
     ap.add_argument("--n", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-c2p", action="store_true", help="skip the learned con2prim leg")
+    ap.add_argument("--c2p-only", action="store_true", help="only the learned con2prim leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of one 384^3 slot (known bytes) for counter calibration")
     ap.add_argument("--precision", default="fp64fast", choices=["fp64", "fp64fast"],
@@ -164,6 +225,10 @@ def main():
             dev.sync()
             print(json.dumps({"copy_bytes_read": dev.slot_bytes(), "copy_bytes_written": dev.slot_bytes(),
                               "copies": 3}), flush=True)
+        return
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.c2p_only:
+        print(json.dumps({"con2prim": run_c2p(log=log)}), flush=True)
         return
     cells = n ** 3
     el, rep, nbad, nresc = run_gpu(n, K, W, ref=False, precision=args.precision)
@@ -219,6 +284,8 @@ def main():
         out["fp32_form"] = {"what": "reconstruction + Riemann in FP32, con2prim + accumulation FP64",
                             "value": cells * max(K // 2, 4) / el3, "unit": "zone-updates/s",
                             "n_unrecovered": nb3}
+    if not args.no_c2p:
+        out["con2prim"] = run_c2p(log=log)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(out), flush=True)
 

@@ -80,6 +80,9 @@ def lib():
             "orc2_run": (L, [p, i, i, C.POINTER(Mode), C.POINTER(i), i, p, p]),
             "orc1_c2p": (None, [p, p, p, p, p, p, L, d, i, d]),
             "orc1_run": (i, [p, p, p, L, d, d, d, i, d, i, i, i, d, p]),
+            "orc4_w0_from_prims": (None, [p, L, d, p]),
+            "orc4_solve": (L, [p, p, L, d, i, d, p, C.POINTER(i), C.POINTER(C.c_uint8)]),
+            "orc4_features": (None, [p, L, C.POINTER(C.c_float), p, p, C.POINTER(C.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_LIB, name)
@@ -318,3 +321,40 @@ def run1(P, dt, dx, T, gamma, rec="minmod", riemann="hll", n_iter=40, tol=1e-6, 
     steps = lib().orc1_run(_p(P[0]), _p(P[1]), _p(P[2]), P.shape[1], dt, dx, T, max_steps, gamma,
                            REC[rec], RIEMANN[riemann], n_iter, tol, C.byref(t))
     return steps, t.value
+
+
+# ---------------------------------------------------------------- batched con2prim (C4)
+def c2p4_w0_from_prims(Pg, gamma):
+    """Initial W of primitive guesses Pg [n][5] as UtoP forms it."""
+    Pg = _f64(Pg)
+    n = Pg.shape[0]
+    W0 = np.empty(n)
+    lib().orc4_w0_from_prims(_p(Pg), n, gamma, _p(W0))
+    return W0
+
+
+def c2p4_solve(U, W0, gamma, n_iter, rtol):
+    """1DW Newton (relative stop, no Gamma cap) from initial W0.
+    Returns (P [n][5], iters [n], converged [n] bool)."""
+    U = _f64(U)
+    W0 = _f64(W0)
+    n = U.shape[0]
+    P = np.full((n, 5), np.nan)
+    it = np.zeros(n, dtype=np.int32)
+    cv = np.zeros(n, dtype=np.uint8)
+    lib().orc4_solve(_p(U), _p(W0), n, gamma, n_iter, rtol, _p(P), it.ctypes.data_as(C.POINTER(C.c_int)),
+                     cv.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return P, it, cv.astype(bool)
+
+
+def c2p4_features(U):
+    """Features X [n][5] (float32), E, z and the domain flag of the learned guess."""
+    U = _f64(U)
+    n = U.shape[0]
+    X = np.empty((n, 5), dtype=np.float32)
+    E = np.empty(n)
+    z = np.empty(n)
+    ok = np.zeros(n, dtype=np.uint8)
+    lib().orc4_features(_p(U), n, X.ctypes.data_as(C.POINTER(C.c_float)), _p(E), _p(z),
+                        ok.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return X, E, z, ok.astype(bool)

@@ -1,0 +1,183 @@
+"""GPU tests of the batched con2prim and the learned guess (include/verona_c2p.h).
+
+* features, plain-guess solves (generic / warm / analytic) bit-exact against
+  the C oracle started from the same initial W;
+* the BF16 MFMA inference kernel: exact on inputs where every sum is exact
+  (pins the lane layout), close to the NumPy BF16 model otherwise;
+* training-kernel gradients against the NumPy reference (FP32 and QAT);
+* training: deterministic, the loss falls, and the trained guess needs fewer
+  Newton iterations than the analytic and generic starts while the
+  recovered P agrees with them to 1e-12 (relative L1 per variable) and with
+  the oracle bit for bit from the same initial W; no unrecovered cells.
+"""
+import numpy as np
+import pytest
+
+import mlp_ref
+from helpers import progress
+from verona_amd._lib import VeronaHIPError
+from verona_amd.c2p import Con2PrimBatch, c4_states, training_set, warm_guess
+
+pytestmark = pytest.mark.gpu
+G = 4.0 / 3.0
+RTOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def cp():
+    with Con2PrimBatch(1 << 20, G) as c:
+        yield c
+
+
+def _rel_l1(a, b):
+    return np.abs(a - b).sum(0) / np.abs(b).sum(0)
+
+
+def test_features_match_oracle(cp, oracle):
+    progress("c2p features")
+    U, P = c4_states(1 << 16, seed=3, gamma=G)
+    X, T, ok = cp.features(U, P)
+    Xo, E, z, oko = oracle.c2p4_features(U)
+    assert (ok <= oko).all() and ok.mean() > 0.99
+    assert np.allclose(X[ok], Xo[ok], rtol=2e-6, atol=2e-6)
+    q = np.log1p((G - 1) * P[:, 1] / E)
+    lor = np.sqrt(1 + (P[:, 2:] ** 2).sum(1))
+    assert np.allclose(T[ok, 0], np.log(q / z)[ok], rtol=1e-6, atol=1e-6)
+    assert np.allclose(T[ok, 1], np.log(lor)[ok], rtol=1e-6, atol=1e-6)
+
+
+def test_mfma_inference_exact_on_exact_data(cp):
+    progress("mfma layout")
+    rng = np.random.default_rng(1)
+    d = {"W1": rng.choice([-1.0, -0.5, 0.0, 0.5, 1.0], (64, 5)), "b1": rng.integers(-4, 5, 64) * 0.25,
+         "W2": rng.choice([-0.5, 0.0, 0.5], (64, 64)), "b2": rng.integers(-4, 5, 64) * 0.25,
+         "W3": rng.choice([-0.25, 0.0, 0.25], (2, 64)), "b3": np.array([0.5, -0.25])}
+    params = mlp_ref.pack(d)
+    X = (rng.integers(-8, 9, (1000, 5)) * 0.25).astype(np.float32)
+    cp.set_params(params)
+    Y = cp.infer(X)
+    Yr = mlp_ref.forward(params, X, qat=True)["y"]
+    assert np.array_equal(Y.astype(np.float64), Yr), np.abs(Y - Yr).max()
+
+
+def test_mfma_inference_matches_bf16_model(cp):
+    progress("mfma inference")
+    U, _ = c4_states(1 << 15, seed=4, gamma=G)
+    X, _, ok = cp.features(U)
+    params = mlp_ref.he_init(11)
+    params[-2:] = [0.1, -0.3]
+    cp.set_params(params)
+    Y = cp.infer(X)
+    Yr = mlp_ref.forward(params, X, qat=True)["y"]
+    err = np.abs(Y - Yr)
+    assert err.max() < 2e-2 * (1 + np.abs(Yr).max()) and err.mean() < 2e-3 * (1 + np.abs(Yr).mean())
+
+
+@pytest.mark.parametrize("qat", [False, True])
+def test_training_gradient_matches_reference(cp, qat):
+    progress(f"grad qat={qat}")
+    U, P = c4_states(4096, seed=5, gamma=G)
+    X, T, ok = cp.features(U, P)
+    X, T = X[ok], T[ok]
+    params = mlp_ref.he_init(12)
+    loss, g = cp.grad(X, T, params, qat=qat)
+    lr, gr = mlp_ref.loss_grad(params, X, T, qat=qat)
+    assert abs(loss - lr) <= 1e-5 * lr
+    tol = 1e-3 if qat else 1e-4
+    assert np.linalg.norm(g - gr) <= tol * np.linalg.norm(gr)
+    # each block of the gradient (W1 b1 W2 b2 W3 b3) on its own
+    o = 0
+    for name, shp in mlp_ref.SHAPES:
+        k = int(np.prod(shp))
+        gb, rb = g[o:o + k], gr[o:o + k]
+        assert np.linalg.norm(gb - rb) <= 10 * tol * (np.linalg.norm(rb) + 1e-12), name
+        o += k
+
+
+@pytest.mark.parametrize("guess", ["generic", "warm", "analytic"])
+def test_plain_guess_bit_exact_vs_oracle(cp, oracle, guess):
+    progress(f"c2p {guess}")
+    U, P = c4_states(1 << 18, seed=6, gamma=G)
+    Pp = warm_guess(P)
+    cp.upload(U, Pp)
+    st = cp.solve(guess, n_iter=100, rtol=RTOL, record=True)
+    Pd, W0, it = cp.download(record=True)
+    if guess == "generic":
+        W0o = np.full(len(U), 1.0 + G)
+    elif guess == "warm":
+        W0o = oracle.c2p4_w0_from_prims(Pp, G)
+    else:
+        W0o = -U[:, 1]
+    assert np.array_equal(W0, W0o)
+    Po, ito, cv = oracle.c2p4_solve(U, W0o, G, 100, RTOL)
+    assert st["n_unrecovered"] == 0 and st["n_fallback"] == int((~cv).sum()) == 0
+    assert np.array_equal(Pd, Po)
+    assert np.array_equal(it.astype(np.int32), ito)
+    assert abs(st["iters_mean"] - ito.mean()) < 1e-9
+
+
+def test_learned_guess_training_and_gain(cp, oracle):
+    progress("c2p training")
+    Ut, Pt = training_set(1 << 18, seed=7, gamma=G)
+    kw = dict(steps=1500, batch=8192, lr=3e-3, seed=7)
+    curve, ms = cp.train(Ut, Pt, **kw)
+    w_a = cp.get_params()
+    assert curve[-100:].mean() < curve[:10].mean() / 20, (curve[:10].mean(), curve[-100:].mean())
+    curve2, _ = cp.train(Ut, Pt, **kw)
+    assert np.array_equal(cp.get_params(), w_a) and np.array_equal(curve, curve2)
+    progress(f"trained in {ms:.0f} ms, loss {curve[-1]:.3g}")
+
+    U, P = c4_states(1 << 20, seed=42, gamma=G)
+    cp.upload(U)
+    sa = cp.solve("analytic", n_iter=100, rtol=RTOL)
+    Pa = cp.download()
+    sg = cp.solve("generic", n_iter=100, rtol=RTOL)
+    sl = cp.solve("learned", n_iter=100, rtol=RTOL, record=True)
+    Pl, W0, it = cp.download(record=True)
+    progress(f"iters mean/wave-max: learned {sl['iters_mean']:.2f}/{sl['wave_max_mean']:.2f} "
+             f"analytic {sa['iters_mean']:.2f}/{sa['wave_max_mean']:.2f} "
+             f"generic {sg['iters_mean']:.2f}/{sg['wave_max_mean']:.2f}")
+    for s in (sa, sg, sl):
+        assert s["n_unrecovered"] == 0
+    assert sl["n_guess_bad"] == 0
+    # same root: recovered P agrees with the plain start to 1e-12
+    assert (_rel_l1(Pl, Pa) <= 1e-12).all(), _rel_l1(Pl, Pa)
+    # and with the oracle from the same initial W, bit for bit
+    Po, ito, cv = oracle.c2p4_solve(U, W0, G, 100, RTOL)
+    assert cv.all() and np.array_equal(Pl, Po) and np.array_equal(it.astype(np.int32), ito)
+    # the gain: fewer Newton iterations than either plain start
+    assert sl["iters_mean"] < sa["iters_mean"]
+    assert sl["wave_max_mean"] < 0.5 * sg["wave_max_mean"]
+    # the guess lies at or above the lower bound E and is closer to W than E
+    E = -U[:, 1]
+    Wt = (P[:, 0] + G * P[:, 1]) * (1 + (P[:, 2:] ** 2).sum(1))
+    assert (W0 >= E).all()
+    assert np.median(np.abs(W0 - Wt) / Wt) < np.median(np.abs(E - Wt) / Wt)
+
+
+def test_learned_guess_ignores_momentum_direction(cp):
+    progress("c2p invariance")
+    U, _ = c4_states(1 << 14, seed=8, gamma=G)
+    cp.set_params(mlp_ref.he_init(13) * 0.5)
+    cp.upload(U)
+    cp.solve("learned", n_iter=100, rtol=RTOL, record=True)
+    _, W0, _ = cp.download(record=True)
+    Uf = U.copy()
+    Uf[:, 2] *= -1.0
+    Uf[:, 4] *= -1.0
+    cp.upload(Uf)
+    cp.solve("learned", n_iter=100, rtol=RTOL, record=True)
+    _, W0f, _ = cp.download(record=True)
+    assert np.array_equal(W0, W0f)
+
+
+def test_error_paths():
+    with Con2PrimBatch(1024, G) as c:
+        U, P = c4_states(100, seed=1, gamma=G)
+        c.upload(U)
+        with pytest.raises(VeronaHIPError, match="trained parameters"):
+            c.solve("learned")
+        with pytest.raises(VeronaHIPError, match="P_prev"):
+            c.solve("warm")
+        with pytest.raises(VeronaHIPError, match="out of range|outside"):
+            c.upload(np.zeros((2048, 5)))

@@ -9,6 +9,8 @@ from . import verona1d as Verona1D
 from . import verona2d as Verona2D
 from . import verona3d as Verona3D
 from .bc import Boundary, Cart
+from .c2p import Con2PrimBatch
 from .device import DeviceGrid, host_array
 
-__all__ = ["EosTypes", "Verona1D", "Verona2D", "Verona3D", "Boundary", "Cart", "DeviceGrid", "host_array"]
+__all__ = ["EosTypes", "Verona1D", "Verona2D", "Verona3D", "Boundary", "Cart", "Con2PrimBatch", "DeviceGrid",
+           "host_array"]

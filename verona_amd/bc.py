@@ -75,3 +75,15 @@ def uniform_stream(seed):
     """Doubles in [0, 1) from the top 53 bits of SplitMix64."""
     for z in splitmix64(seed):
         yield (z >> 11) * (1.0 / 9007199254740992.0)
+
+
+def uniform_array(seed, count):
+    """The first `count` values of uniform_stream(seed), vectorised."""
+    import numpy as np
+    k = np.arange(1, count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)

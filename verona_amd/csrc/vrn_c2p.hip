@@ -1,0 +1,1160 @@
+// vrn_c2p.hip -- batched con2prim with a learned Newton initial guess
+// (gfx950), and the C ABI of include/verona_c2p.h.
+//
+// Recovery.  Every cell runs the 1DW Newton iteration of Verona's UtoP
+// (src/3dim/structs.jl:273-343: W_min clamp, v^2 clamp, halving towards
+// W_min) on W = w Gamma^2, stopped when |dW| < rtol * W.  Only the initial
+// W differs between the guess modes.  A cell the iteration leaves unfinished
+// goes through the rest of the reference cascade (c2p_cascade_tail).
+//
+// Learned guess.  For an ideal gas W/E (E = -T^t_t) depends only on D/E and
+// |S|/E, and W = E + p.  The network predicts y0 = log(q / z) with
+// q = log(W/E) and z = (E^2 - |S|^2 - D^2) / E^2 >= 0, which to first order
+// in p is 2 rho p Gamma^2 / ((gamma - 1) E^2): y0 stays O(1) over the whole
+// state space, and the guess W0 = E (1 + expm1(z e^y0)) never falls below the
+// lower bound E, so the relative guess error is (error of y0) * q -- small
+// exactly where the reference's W_min clamp makes Newton slow (Gamma >> 1).
+// The second output is log Gamma.  Inputs: log D, log tau (tau = E - D),
+// log1p(|S|/tau), |S|/E and sqrt(z); none depends on the direction of S.
+//
+// Inference runs inside the solve kernel: a wave takes 64 cells, computes
+// their features, and evaluates the 5 -> 64 -> 64 layers with
+// v_mfma_f32_16x16x16_bf16 in the transposed orientation H^T = W X^T, so the
+// 16x16 accumulator tile of one layer (rows = hidden units, columns = cells)
+// is, element for element, the B operand of the next layer: no LDS round trip
+// between layers.  The 64 -> 2 output layer is 32 FP32 FMAs per lane plus a
+// 4-way cross-lane sum, so the second hidden layer is never rounded to BF16.
+//
+// Training (FP32 VALU): one thread per sample runs forward and backward with
+// the parameters in LDS and caches per-sample activations and deltas; weight
+// gradients are sums over samples, formed by a thread per parameter over a
+// fixed slice of the batch and then a fixed-order sum over slices, so the
+// result is deterministic; Adam runs one thread per parameter.
+#include "../../include/verona_c2p.h"
+#define VRN_NS vrn_c4
+#include "vrn_physics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace vrn_c4 {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NIN = VRN_MLP_NIN, NH = VRN_MLP_NHID, NOUT = VRN_MLP_NOUT, NPARAM = VRN_MLP_NPARAM;
+constexpr int OFF_W1 = 0, OFF_B1 = NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NOUT * NH;
+static_assert(OFF_B3 + NOUT == NPARAM, "parameter packing");
+
+constexpr int RECORD_FLAG = 0x100;
+
+// ---------------------------------------------------------------- BF16
+// Round to nearest even (the rounding of v_cvt_pk_bf16_f32); NaN stays NaN.
+__host__ __device__ __forceinline__ unsigned short bf16_bits(float f)
+{
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_float(unsigned short b)
+{
+    return __builtin_bit_cast(float, (unsigned)b << 16);
+}
+__host__ __device__ __forceinline__ float bf16_round(float f) { return bf16_float(bf16_bits(f)); }
+
+// ---------------------------------------------------------------- features
+struct Feat {
+    float x[NIN];
+    double E, z;
+    bool ok;
+};
+
+// Features of one cell of conserved variables (D, T^t_t, S^2).
+__device__ __forceinline__ Feat c4_features(double D, double Qt, double S2)
+{
+    Feat f;
+    f.E = -Qt;
+    const double tau = f.E - D;
+    const double Sm = __builtin_sqrt(S2);
+    f.z = (tau * (f.E + D) - S2) / (f.E * f.E);
+    f.ok = D > 0.0 && tau > 0.0 && f.z > 0.0 && f.z < 1.0 && __builtin_isfinite(f.E) &&
+           __builtin_isfinite(S2);
+    if (f.ok) {
+        const float tf = (float)tau;
+        f.x[0] = 0.1f * __logf((float)D);
+        f.x[1] = 0.1f * __logf(tf);
+        f.x[2] = 0.2f * log1pf((float)Sm / tf);
+        f.x[3] = (float)(Sm / f.E);
+        f.x[4] = __builtin_sqrtf((float)f.z);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NIN; k++) f.x[k] = 0.0f;
+    }
+    return f;
+}
+
+// Training targets (log(q/z), log Gamma) of a (U, P) pair; false outside the
+// model's domain.
+__device__ __forceinline__ bool c4_targets(const double *P, double g, const Feat &f, float *t)
+{
+    const double p = (g - 1.0) * P[1];
+    const double gam = __builtin_sqrt(1.0 + P[2] * P[2] + P[3] * P[3] + P[4] * P[4]);
+    const double q = log1p(p / f.E);
+    const double t0 = log(q / f.z), t1 = log(gam);
+    t[0] = (float)t0;
+    t[1] = (float)t1;
+    return f.ok && q > 0.0 && __builtin_isfinite(t0) && __builtin_isfinite(t1);
+}
+
+// Initial W from the network's first output.
+__device__ __forceinline__ double learned_w0(const Feat &f, float y0)
+{
+    float qh = (float)f.z * __expf(y0);
+    qh = qh < 0.0f ? 0.0f : (qh > 0.5f ? 0.5f : qh);
+    return f.E + f.E * (double)expm1f(qh);
+}
+
+// ---------------------------------------------------------------- Newton
+// 1DW iteration of src/3dim/structs.jl:292-342 with the relative stop
+// |dW| < rtol * W.  W holds the initial guess on entry and the root on exit.
+__device__ __forceinline__ bool newton_1dw_rel(double D, double Qt, double S2, double g, int n_iter,
+                                               double rtol, double &W, int &iters)
+{
+    const double c = (g - 1.0) / g;
+    const double W_max = 1e30;
+    const double W_min = __builtin_sqrt(S2) * (1.0 + 1e-15);
+    for (int t10 = 0; t10 < 1100 && S2 / (W * W) >= 1.0 && W < W_max; t10++) W *= 10.0;
+    for (int it = 0; it < n_iter; it++) {
+        iters++;
+        if (W < W_min) W = W_min;
+        double v2 = S2 / (W * W);
+        if (v2 < 0.0)
+            v2 = 0.0;
+        else if (v2 > 1.0 - 1e-15)
+            v2 = 1.0 - 1e-15;
+        const double sv = __builtin_sqrt(1.0 - v2);
+        const double fun = Qt + W - c * (W * (1.0 - v2) - D * sv);
+        const double W3 = W * W * W;
+        const double jac = 1.0 - c * (1.0 - v2) - c * (W * (2.0 * S2 / W3) - D * (S2 / (W3 * sv)));
+        double dW = fun / jac;
+        const double Wp = W - dW;
+        W = (Wp < W_min) ? 0.5 * (W + W_min) : Wp;
+        if (dW < 0.0) dW = -dW;
+        if (dW < rtol * W) return true;
+    }
+    return false;
+}
+
+// Primitives from the converged W (src/3dim/structs.jl:598-609 without the
+// Gamma <= 50 cap).
+__device__ __forceinline__ void recover(double D, double Qx, double Qy, double Qz, double S2, double W,
+                                        double g, double *P)
+{
+    const double v2 = S2 / (W * W);
+    const double gam = 1.0 / __builtin_sqrt(1.0 - v2);
+    P[0] = jmax(1e-8, D / gam);
+    P[1] = jmax(1e-8, (W / (gam * gam) - D / gam) / g);
+    P[2] = gam * Qx / W;
+    P[3] = gam * Qy / W;
+    P[4] = gam * Qz / W;
+}
+
+// ---------------------------------------------------------------- wave helpers
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+struct Counters {
+    unsigned long long n_unrecovered, n_fallback, n_guess_bad, iters_sum, wave_max_sum, n_waves;
+    unsigned long long iters_max;
+    unsigned long long hist[64];
+    unsigned long long n_rescue;  // entries in the rescue list
+};
+
+struct Rescue {
+    long i;     // cell
+    double W0;  // its initial W
+};
+
+struct SolveArgs {
+    long n;
+    const double *U;      // SoA: plane v at U + v * n
+    const double *Pprev;  // SoA (warm guess)
+    double *P;            // SoA output
+    double *W0;           // per-cell initial W (record mode)
+    unsigned char *iters; // per-cell iterations (record mode)
+    Counters *cnt;
+    Rescue *rescue;       // cells the 1DW iteration did not finish
+    double g, rtol;
+    int n_iter;
+    int record;
+    const float *params;  // FP32 parameters (learned guess)
+};
+
+// Weights of the learned guess as one wave holds them.
+struct NetRegs {
+    bf16x4 w1[4];      // A fragments of W1 (16 x 16 tiles, K padded 5 -> 16)
+    bf16x4 w2[4][4];   // A fragments of W2, [row tile][k tile]
+};
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
+{
+    bf16x4 r;
+    r[0] = (short)bf16_bits(a);
+    r[1] = (short)bf16_bits(b);
+    r[2] = (short)bf16_bits(c);
+    r[3] = (short)bf16_bits(d);
+    return r;
+}
+
+// Lane l of a 16x16x16 MFMA supplies A[m = l % 16][k = 4 (l / 16) + j] and
+// B[k = 4 (l / 16) + j][n = l % 16] (j = 0..3) and holds
+// D[m = 4 (l / 16) + j][n = l % 16].
+__device__ __forceinline__ void load_net(const float *prm, NetRegs &nr, int lane)
+{
+    const int m = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = kq + j;
+            v[j] = k < NIN ? prm[OFF_W1 + (16 * r + m) * NIN + k] : 0.0f;
+        }
+        nr.w1[r] = pack4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) {
+            const float *w = prm + OFF_W2 + (16 * r + m) * NH + 16 * kt + kq;
+            nr.w2[r][kt] = pack4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+// Shared copy of the small per-unit parameters (b1, b2, BF16-rounded W3, b3).
+struct NetShared {
+    float b1[NH], b2[NH], w3[NOUT][NH], b3[NOUT];
+};
+
+__device__ __forceinline__ void load_net_shared(const float *prm, NetShared &ns)
+{
+    for (int i = threadIdx.x; i < NH; i += blockDim.x) {
+        ns.b1[i] = prm[OFF_B1 + i];
+        ns.b2[i] = prm[OFF_B2 + i];
+        ns.w3[0][i] = bf16_round(prm[OFF_W3 + i]);
+        ns.w3[1][i] = bf16_round(prm[OFF_W3 + NH + i]);
+    }
+    if (threadIdx.x < NOUT) ns.b3[threadIdx.x] = prm[OFF_B3 + threadIdx.x];
+}
+
+// Network outputs for the 64 cells of a wave; lane c passes the features of
+// cell c and receives that cell's outputs.  All 64 lanes must be active.
+__device__ __forceinline__ void net_eval(const NetRegs &nr, const NetShared &ns, const float *x,
+                                         int lane, float &y0, float &y1)
+{
+    const int q = lane >> 4, kq = 4 * q;
+    const unsigned p01 = (unsigned)bf16_bits(x[0]) | ((unsigned)bf16_bits(x[1]) << 16);
+    const unsigned p23 = (unsigned)bf16_bits(x[2]) | ((unsigned)bf16_bits(x[3]) << 16);
+    const unsigned p4 = (unsigned)bf16_bits(x[4]);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int src = 16 * t + (lane & 15);
+        const unsigned a01 = (unsigned)__shfl((int)p01, src);
+        const unsigned a23 = (unsigned)__shfl((int)p23, src);
+        const unsigned a4 = (unsigned)__shfl((int)p4, src);
+        bf16x4 bx;
+        if (q == 0) {
+            bx[0] = (short)(a01 & 0xffff);
+            bx[1] = (short)(a01 >> 16);
+            bx[2] = (short)(a23 & 0xffff);
+            bx[3] = (short)(a23 >> 16);
+        } else {
+            bx[0] = (short)(q == 1 ? a4 : 0u);
+            bx[1] = 0;
+            bx[2] = 0;
+            bx[3] = 0;
+        }
+        bf16x4 h1[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            f32x4 acc;
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] = ns.b1[16 * r + kq + j];
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w1[r], bx, acc, 0, 0, 0);
+            h1[r] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f),
+                          fmaxf(acc[3], 0.0f));
+        }
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int r2 = 0; r2 < 4; r2++) {
+            f32x4 acc;
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] = ns.b2[16 * r2 + kq + j];
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w2[r2][kt], h1[kt], acc, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float h = fmaxf(acc[j], 0.0f);
+                s0 = s0 + ns.w3[0][16 * r2 + kq + j] * h;
+                s1 = s1 + ns.w3[1][16 * r2 + kq + j] * h;
+            }
+        }
+        // sum the four row groups: lanes l, l^16, l^32, l^48 (same cell)
+        s0 = s0 + __shfl_xor(s0, 16);
+        s1 = s1 + __shfl_xor(s1, 16);
+        s0 = s0 + __shfl_xor(s0, 32);
+        s1 = s1 + __shfl_xor(s1, 32);
+        // lane l now holds the outputs of cell 16 t + (l & 15); it is its own
+        // cell when t == l / 16
+        if (t == q) {
+            y0 = s0 + ns.b3[0];
+            y1 = s1 + ns.b3[1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- solve
+template <int GUESS>
+__global__ void __launch_bounds__(256) k_solve(SolveArgs a)
+{
+    __shared__ NetShared ns;
+    __shared__ unsigned hist[64];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) hist[i] = 0;
+    NetRegs nr;
+    if constexpr (GUESS == VRN_GUESS_LEARNED) {
+        load_net_shared(a.params, ns);
+        load_net(a.params, nr, lane);
+    }
+    __syncthreads();
+    const long n = a.n;
+    const long nchunks = (n + 63) / 64;
+    unsigned long long unrec = 0, fall = 0, bad = 0, isum = 0, wmax_sum = 0, nw = 0;
+    int imax = 0;
+    for (long ch = (long)blockIdx.x * 4 + wid; ch < nchunks; ch += (long)gridDim.x * 4) {
+        const long i = ch * 64 + lane;
+        const bool act = i < n;
+        double u[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) u[v] = act ? a.U[v * n + i] : 0.0;
+        const double S2 = u[2] * u[2] + u[3] * u[3] + u[4] * u[4];
+        double W;
+        double Pg[5];
+        if constexpr (GUESS == VRN_GUESS_GENERIC) {
+            Pg[0] = 1.0;
+            Pg[1] = 1.0;
+            Pg[2] = Pg[3] = Pg[4] = 0.0;
+        } else if constexpr (GUESS == VRN_GUESS_WARM) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) Pg[v] = act ? a.Pprev[v * n + i] : 1.0;
+        }
+        if constexpr (GUESS == VRN_GUESS_GENERIC || GUESS == VRN_GUESS_WARM) {
+            // W from the guess as UtoP forms it (src/3dim/structs.jl:276-287)
+            const double gm = __builtin_sqrt(Pg[2] * Pg[2] + Pg[3] * Pg[3] + Pg[4] * Pg[4] + 1.0);
+            W = (Pg[0] + a.g * Pg[1]) * (gm * gm);
+        } else if constexpr (GUESS == VRN_GUESS_ANALYTIC) {
+            W = -u[1];
+        } else {
+            const Feat f = c4_features(u[0], u[1], S2);
+            float y0 = 0.0f, y1 = 0.0f;
+            net_eval(nr, ns, f.x, lane, y0, y1);
+            (void)y1;
+            if (f.ok && __builtin_isfinite(y0)) {
+                W = learned_w0(f, y0);
+            } else {
+                W = f.E;
+                if (act) bad++;
+            }
+        }
+        const double W0 = W;
+        int it = 0;
+        if (act) {
+            const bool conv = newton_1dw_rel(u[0], u[1], S2, a.g, a.n_iter, a.rtol, W, it);
+            if (conv) {
+                double P[5];
+                recover(u[0], u[2], u[3], u[4], S2, W, a.g, P);
+                bool fin = true;
+#pragma unroll
+                for (int v = 0; v < 5; v++) {
+                    fin = fin && __builtin_isfinite(P[v]);
+                    a.P[v * n + i] = P[v];
+                }
+                if (!fin) unrec++;
+            } else {
+                fall++;
+                const unsigned long long r = atomicAdd(&a.cnt->n_rescue, 1ull);
+                a.rescue[r].i = i;
+                a.rescue[r].W0 = W0;
+            }
+            if (a.record) {
+                a.W0[i] = W0;
+                a.iters[i] = (unsigned char)(it > 255 ? 255 : it);
+            }
+            isum += it;
+            imax = it > imax ? it : imax;
+            atomicAdd(&hist[it < 63 ? it : 63], 1u);
+        }
+        const int wm = wave_max_i(act ? it : 0);
+        if (lane == 0) {
+            wmax_sum += wm;
+            nw++;
+        }
+    }
+    // per-wave totals, then one atomic per wave
+    const long long t_unrec = wave_sum_ll((long long)unrec), t_fall = wave_sum_ll((long long)fall),
+                    t_bad = wave_sum_ll((long long)bad), t_isum = wave_sum_ll((long long)isum);
+    const int t_imax = wave_max_i(imax);
+    if (lane == 0) {
+        if (t_unrec) atomicAdd(&a.cnt->n_unrecovered, (unsigned long long)t_unrec);
+        if (t_fall) atomicAdd(&a.cnt->n_fallback, (unsigned long long)t_fall);
+        if (t_bad) atomicAdd(&a.cnt->n_guess_bad, (unsigned long long)t_bad);
+        atomicAdd(&a.cnt->iters_sum, (unsigned long long)t_isum);
+        atomicAdd(&a.cnt->wave_max_sum, wmax_sum);
+        atomicAdd(&a.cnt->n_waves, nw);
+        atomicMax(&a.cnt->iters_max, (unsigned long long)t_imax);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 64; b += blockDim.x)
+        if (hist[b]) atomicAdd(&a.cnt->hist[b], (unsigned long long)hist[b]);
+}
+
+// Cells the 1DW iteration left unfinished: the rest of the reference cascade
+// (2D Newton, 5D Newton, bisection) from the primitive guess of the mode.
+template <int GUESS>
+__global__ void __launch_bounds__(256) k_rescue(SolveArgs a)
+{
+    const long nr = (long)a.cnt->n_rescue;
+    const long n = a.n;
+    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < nr; r += (long)gridDim.x * blockDim.x) {
+        const long i = a.rescue[r].i;
+        double u[5], P[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) u[v] = a.U[v * n + i];
+        if constexpr (GUESS == VRN_GUESS_GENERIC) {
+            P[0] = 1.0;
+            P[1] = 1.0;
+            P[2] = P[3] = P[4] = 0.0;
+        } else if constexpr (GUESS == VRN_GUESS_WARM) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) P[v] = a.Pprev[v * n + i];
+        } else {
+            // a state at rest with the guessed W
+            P[0] = u[0];
+            P[1] = jmax(1e-8, (a.rescue[r].W0 - u[0]) / a.g);
+            P[2] = P[3] = P[4] = 0.0;
+        }
+        int it = 0;
+        c2p_cascade_tail(u, P, a.g, a.n_iter, a.rtol * (-u[1]), it);
+        bool fin = true;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            fin = fin && __builtin_isfinite(P[v]);
+            a.P[v * n + i] = P[v];
+        }
+        if (!fin) atomicAdd(&a.cnt->n_unrecovered, 1ull);
+        atomicAdd(&a.cnt->iters_sum, (unsigned long long)it);
+    }
+}
+
+// ---------------------------------------------------------------- data movement
+__global__ void k_aos_to_soa(long n, const double *__restrict__ src, double *__restrict__ dst)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < 5 * n; t += (long)gridDim.x * blockDim.x) {
+        const long i = t / 5, v = t % 5;
+        dst[v * n + i] = src[t];
+    }
+}
+__global__ void k_soa_to_aos(long n, const double *__restrict__ src, double *__restrict__ dst)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < 5 * n; t += (long)gridDim.x * blockDim.x) {
+        const long i = t / 5, v = t % 5;
+        dst[t] = src[v * n + i];
+    }
+}
+__global__ void k_zero_counters(Counters *c)
+{
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(c);
+    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
+}
+
+// Features (and targets) of AoS (U, P) pairs.
+__global__ void k_features(long n, const double *__restrict__ U, const double *__restrict__ P, double g,
+                           float *__restrict__ X, float *__restrict__ T, unsigned char *__restrict__ valid)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double *u = U + 5 * i;
+        const double S2 = u[2] * u[2] + u[3] * u[3] + u[4] * u[4];
+        const Feat f = c4_features(u[0], u[1], S2);
+#pragma unroll
+        for (int k = 0; k < NIN; k++) X[NIN * i + k] = f.x[k];
+        bool ok = f.ok;
+        if (P) {
+            float t[2];
+            ok = c4_targets(P + 5 * i, g, f, t);
+            T[2 * i] = t[0];
+            T[2 * i + 1] = t[1];
+        }
+        valid[i] = ok ? 1 : 0;
+    }
+}
+
+// Inference of the network alone on given features (64 samples per wave).
+__global__ void __launch_bounds__(256) k_infer(long n, const float *__restrict__ X, const float *prm,
+                                               float *__restrict__ Y)
+{
+    __shared__ NetShared ns;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    NetRegs nr;
+    load_net_shared(prm, ns);
+    load_net(prm, nr, lane);
+    __syncthreads();
+    const long nchunks = (n + 63) / 64;
+    for (long ch = (long)blockIdx.x * 4 + wid; ch < nchunks; ch += (long)gridDim.x * 4) {
+        const long i = ch * 64 + lane;
+        float x[NIN];
+#pragma unroll
+        for (int k = 0; k < NIN; k++) x[k] = i < n ? X[NIN * i + k] : 0.0f;
+        float y0 = 0.0f, y1 = 0.0f;
+        net_eval(nr, ns, x, lane, y0, y1);
+        if (i < n) {
+            Y[2 * i] = y0;
+            Y[2 * i + 1] = y1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- training
+// Per-sample cache layout (floats): x[8] h1[64] h2[64] dz1[64] dz2[64] dy[2] loss[1]
+constexpr int C_X = 0, C_H1 = 8, C_H2 = C_H1 + NH, C_DZ1 = C_H2 + NH, C_DZ2 = C_DZ1 + NH,
+              C_DY = C_DZ2 + NH, C_L = C_DY + NOUT, C_STRIDE = 272;
+static_assert(C_L < C_STRIDE, "cache stride");
+
+__device__ __forceinline__ unsigned long long splitmix(unsigned long long x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+struct TrainArgs {
+    const float *X, *T;
+    const int *idx;      // valid-sample list (sampling) or NULL (identity)
+    long n_idx;
+    int B;
+    unsigned long long seed;
+    int step;
+    const float *prm;
+    float *cache;
+    float w1;
+    int qat;
+};
+
+__global__ void __launch_bounds__(256) k_train_fwd_bwd(TrainArgs a)
+{
+    __shared__ float w[NPARAM];
+    for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) {
+        float v = a.prm[p];
+        const bool is_w = p < OFF_B1 || (p >= OFF_W2 && p < OFF_B2) || (p >= OFF_W3 && p < OFF_B3);
+        w[p] = (a.qat && is_w) ? bf16_round(v) : v;
+    }
+    __syncthreads();
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.B) return;
+    long id = s;
+    if (a.idx) {
+        unsigned long long r = splitmix(a.seed ^ ((unsigned long long)a.step * 0x100000001B3ull + (unsigned long long)s));
+        id = a.idx[(long)(r % (unsigned long long)a.n_idx)];
+    }
+    float *c = a.cache + (long)s * C_STRIDE;
+    float x[NIN];
+#pragma unroll
+    for (int k = 0; k < NIN; k++) {
+        x[k] = a.X[(long)NIN * id + k];
+        if (a.qat) x[k] = bf16_round(x[k]);
+        c[C_X + k] = x[k];
+    }
+    float h1[NH];
+#pragma unroll
+    for (int j = 0; j < NH; j++) {
+        float z = w[OFF_B1 + j];
+#pragma unroll
+        for (int k = 0; k < NIN; k++) z = z + w[OFF_W1 + j * NIN + k] * x[k];
+        float h = fmaxf(z, 0.0f);
+        if (a.qat) h = bf16_round(h);
+        h1[j] = h;
+        c[C_H1 + j] = h;
+    }
+    float y0 = w[OFF_B3], y1 = w[OFF_B3 + 1];
+    for (int j = 0; j < NH; j++) {
+        float z = w[OFF_B2 + j];
+#pragma unroll
+        for (int k = 0; k < NH; k++) z = z + w[OFF_W2 + j * NH + k] * h1[k];
+        const float h = fmaxf(z, 0.0f);
+        c[C_H2 + j] = h;
+        y0 = y0 + w[OFF_W3 + j] * h;
+        y1 = y1 + w[OFF_W3 + NH + j] * h;
+    }
+    const float e0 = y0 - a.T[2 * id], e1 = y1 - a.T[2 * id + 1];
+    const float invB = 1.0f / (float)a.B;
+    const float dy0 = 2.0f * e0 * invB, dy1 = 2.0f * a.w1 * e1 * invB;
+    c[C_DY] = dy0;
+    c[C_DY + 1] = dy1;
+    c[C_L] = (e0 * e0 + a.w1 * e1 * e1) * invB;
+    // dz2 = (W3^T dy) * [z2 > 0]   (h2 > 0 <=> z2 > 0)
+    float dh1[NH];
+#pragma unroll
+    for (int k = 0; k < NH; k++) dh1[k] = 0.0f;
+    for (int j = 0; j < NH; j++) {
+        const float h = c[C_H2 + j];
+        const float d = h > 0.0f ? (w[OFF_W3 + j] * dy0 + w[OFF_W3 + NH + j] * dy1) : 0.0f;
+        c[C_DZ2 + j] = d;
+#pragma unroll
+        for (int k = 0; k < NH; k++) dh1[k] = dh1[k] + w[OFF_W2 + j * NH + k] * d;
+    }
+#pragma unroll
+    for (int k = 0; k < NH; k++) c[C_DZ1 + k] = h1[k] > 0.0f ? dh1[k] : 0.0f;
+}
+
+// Partial gradient sums: thread = parameter (+1 for the loss), blockIdx.y =
+// slice of the batch.  part[slice][NPARAM + 1].
+__global__ void __launch_bounds__(256) k_train_grad_partial(const float *__restrict__ cache, int B, int nsplit,
+                                                            float *__restrict__ part)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > NPARAM) return;
+    const int per = (B + nsplit - 1) / nsplit;
+    const int s0 = blockIdx.y * per, s1 = min(B, s0 + per);
+    int oa, ob;  // cache offsets of the two factors (ob < 0: factor 1)
+    if (p < OFF_B1) { oa = C_DZ1 + p / NIN; ob = C_X + p % NIN; }
+    else if (p < OFF_W2) { oa = C_DZ1 + (p - OFF_B1); ob = -1; }
+    else if (p < OFF_B2) { oa = C_DZ2 + (p - OFF_W2) / NH; ob = C_H1 + (p - OFF_W2) % NH; }
+    else if (p < OFF_W3) { oa = C_DZ2 + (p - OFF_B2); ob = -1; }
+    else if (p < OFF_B3) { oa = C_DY + (p - OFF_W3) / NH; ob = C_H2 + (p - OFF_W3) % NH; }
+    else if (p < NPARAM) { oa = C_DY + (p - OFF_B3); ob = -1; }
+    else { oa = C_L; ob = -1; }
+    float acc = 0.0f;
+    if (ob >= 0) {
+        for (int s = s0; s < s1; s++) acc = acc + cache[(long)s * C_STRIDE + oa] * cache[(long)s * C_STRIDE + ob];
+    } else {
+        for (int s = s0; s < s1; s++) acc = acc + cache[(long)s * C_STRIDE + oa];
+    }
+    part[(long)blockIdx.y * (NPARAM + 1) + p] = acc;
+}
+
+__global__ void k_train_grad_final(const float *__restrict__ part, int nsplit, float *__restrict__ grad)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > NPARAM) return;
+    float acc = 0.0f;
+    for (int s = 0; s < nsplit; s++) acc = acc + part[(long)s * (NPARAM + 1) + p];
+    grad[p] = acc;
+}
+
+__global__ void k_adam(float *__restrict__ prm, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ grad, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2, float *loss_out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == NPARAM && loss_out) *loss_out = grad[NPARAM];
+    if (p >= NPARAM) return;
+    const float g = grad[p];
+    const float mm = b1 * m[p] + (1.0f - b1) * g;
+    const float vv = b2 * v[p] + (1.0f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    prm[p] = prm[p] - lr * (mm / bc1) / (__builtin_sqrtf(vv / bc2) + eps);
+}
+
+}  // namespace vrn_c4
+
+// ================================================================ host
+using namespace vrn_c4;
+
+namespace {
+thread_local std::string t_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(-2, "%s: %s", #x, hipGetErrorString(e_));            \
+    } while (0)
+
+int grid_for(long work, int block, int cap = 8192)
+{
+    long g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+constexpr int MAX_BATCH = 65536;
+constexpr int NSPLIT = 64;
+}  // namespace
+
+struct vrn_c2p {
+    long n_max = 0, n = 0;
+    double g = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double *U = nullptr, *Pprev = nullptr, *P = nullptr, *W0 = nullptr, *stage = nullptr;
+    unsigned char *iters = nullptr;
+    Counters *cnt = nullptr;
+    Rescue *rescue = nullptr;
+    float *params = nullptr;  // device FP32 parameters
+    bool have_params = false, have_prev = false, have_P = false, recorded = false;
+    // training / test buffers (allocated on first use)
+    float *cache = nullptr, *part = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr, *tprm = nullptr;
+    float *X = nullptr, *T = nullptr;
+    int *idx = nullptr;
+    unsigned char *valid = nullptr;
+    double *pairU = nullptr, *pairP = nullptr;
+    long train_cap = 0;
+};
+
+namespace {
+
+int ensure_train(vrn_c2p *h)
+{
+    if (h->cache) return 0;
+    HIPCHK(hipMalloc(&h->cache, sizeof(float) * (size_t)C_STRIDE * MAX_BATCH));
+    HIPCHK(hipMalloc(&h->part, sizeof(float) * (size_t)NSPLIT * (NPARAM + 1)));
+    HIPCHK(hipMalloc(&h->grad, sizeof(float) * (NPARAM + 1)));
+    HIPCHK(hipMalloc(&h->m, sizeof(float) * NPARAM));
+    HIPCHK(hipMalloc(&h->v, sizeof(float) * NPARAM));
+    HIPCHK(hipMalloc(&h->tprm, sizeof(float) * NPARAM));
+    return 0;
+}
+
+int ensure_pairs(vrn_c2p *h, long n)
+{
+    if (n <= h->train_cap) return 0;
+    (void)hipFree(h->X);
+    (void)hipFree(h->T);
+    (void)hipFree(h->idx);
+    (void)hipFree(h->valid);
+    (void)hipFree(h->pairU);
+    (void)hipFree(h->pairP);
+    h->X = h->T = nullptr;
+    h->idx = nullptr;
+    h->valid = nullptr;
+    h->pairU = h->pairP = nullptr;
+    h->train_cap = 0;
+    HIPCHK(hipMalloc(&h->X, sizeof(float) * NIN * (size_t)n));
+    HIPCHK(hipMalloc(&h->T, sizeof(float) * 2 * (size_t)n));
+    HIPCHK(hipMalloc(&h->idx, sizeof(int) * (size_t)n));
+    HIPCHK(hipMalloc(&h->valid, (size_t)n));
+    HIPCHK(hipMalloc(&h->pairU, sizeof(double) * 5 * (size_t)n));
+    HIPCHK(hipMalloc(&h->pairP, sizeof(double) * 5 * (size_t)n));
+    h->train_cap = n;
+    return 0;
+}
+
+// One forward/backward + gradient reduction over B samples (cache -> grad).
+int grad_pass(vrn_c2p *h, const float *X, const float *T, const int *idx, long n_idx, int B,
+              unsigned long long seed, int step, const float *prm, float w1, int qat)
+{
+    TrainArgs ta;
+    ta.X = X;
+    ta.T = T;
+    ta.idx = idx;
+    ta.n_idx = n_idx;
+    ta.B = B;
+    ta.seed = seed;
+    ta.step = step;
+    ta.prm = prm;
+    ta.cache = h->cache;
+    ta.w1 = w1;
+    ta.qat = qat;
+    hipLaunchKernelGGL(k_train_fwd_bwd, dim3((B + 255) / 256), dim3(256), 0, h->s, ta);
+    const int nsplit = B < NSPLIT ? B : NSPLIT;
+    hipLaunchKernelGGL(k_train_grad_partial, dim3((NPARAM + 1 + 255) / 256, nsplit), dim3(256), 0, h->s,
+                       (const float *)h->cache, B, nsplit, h->part);
+    hipLaunchKernelGGL(k_train_grad_final, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, h->s,
+                       (const float *)h->part, nsplit, h->grad);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Seeded He initialisation on the host (SplitMix64 + Box-Muller).
+void he_init(std::vector<float> &w, unsigned long long seed)
+{
+    unsigned long long st = seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        unsigned long long z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto normal = [&]() {
+        double u1 = next(), u2 = next();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    w.assign(NPARAM, 0.0f);
+    for (int i = 0; i < NH * NIN; i++) w[OFF_W1 + i] = (float)(normal() * std::sqrt(2.0 / NIN));
+    for (int i = 0; i < NH * NH; i++) w[OFF_W2 + i] = (float)(normal() * std::sqrt(2.0 / NH));
+    for (int i = 0; i < NOUT * NH; i++) w[OFF_W3 + i] = (float)(normal() * std::sqrt(1.0 / NH));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vrn_c2p_last_error(void) { return t_err.c_str(); }
+
+int vrn_c2p_create(int64_t n_max, double gamma, vrn_c2p **out)
+{
+    if (!out) return fail(-1, "vrn_c2p_create: out is NULL");
+    *out = nullptr;
+    if (n_max < 1 || n_max > (int64_t)1 << 31) return fail(-1, "vrn_c2p_create: n_max %lld out of range", (long long)n_max);
+    if (!(gamma > 1.0 && gamma <= 2.0)) return fail(-1, "vrn_c2p_create: gamma %g outside (1, 2]", gamma);
+    vrn_c2p *h = new vrn_c2p();
+    h->n_max = n_max;
+    h->g = gamma;
+    auto cleanup = [&](int rc) {
+        vrn_c2p_destroy(h);
+        return rc;
+    };
+    const size_t nb = sizeof(double) * 5 * (size_t)n_max;
+    if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess ||
+        hipMalloc(&h->U, nb) != hipSuccess || hipMalloc(&h->P, nb) != hipSuccess ||
+        hipMalloc(&h->stage, nb) != hipSuccess || hipMalloc(&h->W0, sizeof(double) * n_max) != hipSuccess ||
+        hipMalloc(&h->iters, n_max) != hipSuccess || hipMalloc(&h->cnt, sizeof(Counters)) != hipSuccess ||
+        hipMalloc(&h->rescue, sizeof(Rescue) * n_max) != hipSuccess ||
+        hipMalloc(&h->params, sizeof(float) * NPARAM) != hipSuccess)
+        return cleanup(fail(-2, "vrn_c2p_create: device allocation of %zu bytes failed", 4 * nb));
+    *out = h;
+    return 0;
+}
+
+int vrn_c2p_destroy(vrn_c2p *h)
+{
+    if (!h) return 0;
+    if (h->s) (void)hipStreamSynchronize(h->s);
+    (void)hipFree(h->U);
+    (void)hipFree(h->Pprev);
+    (void)hipFree(h->P);
+    (void)hipFree(h->W0);
+    (void)hipFree(h->stage);
+    (void)hipFree(h->iters);
+    (void)hipFree(h->cnt);
+    (void)hipFree(h->rescue);
+    (void)hipFree(h->params);
+    (void)hipFree(h->cache);
+    (void)hipFree(h->part);
+    (void)hipFree(h->grad);
+    (void)hipFree(h->m);
+    (void)hipFree(h->v);
+    (void)hipFree(h->tprm);
+    (void)hipFree(h->X);
+    (void)hipFree(h->T);
+    (void)hipFree(h->idx);
+    (void)hipFree(h->valid);
+    (void)hipFree(h->pairU);
+    (void)hipFree(h->pairP);
+    if (h->e0) (void)hipEventDestroy(h->e0);
+    if (h->e1) (void)hipEventDestroy(h->e1);
+    if (h->s) (void)hipStreamDestroy(h->s);
+    delete h;
+    return 0;
+}
+
+int vrn_c2p_upload(vrn_c2p *h, int64_t n, const double *U, const double *P_prev)
+{
+    if (!h || !U) return fail(-1, "vrn_c2p_upload: NULL argument");
+    if (n < 1 || n > h->n_max) return fail(-1, "vrn_c2p_upload: n = %lld outside [1, %ld]", (long long)n, h->n_max);
+    const size_t nb = sizeof(double) * 5 * (size_t)n;
+    HIPCHK(hipMemcpyAsync(h->stage, U, nb, hipMemcpyHostToDevice, h->s));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(5 * n, 256)), dim3(256), 0, h->s, (long)n,
+                       (const double *)h->stage, h->U);
+    HIPCHK(hipGetLastError());
+    h->have_prev = false;
+    if (P_prev) {
+        if (!h->Pprev) HIPCHK(hipMalloc(&h->Pprev, sizeof(double) * 5 * (size_t)h->n_max));
+        HIPCHK(hipStreamSynchronize(h->s));
+        HIPCHK(hipMemcpyAsync(h->stage, P_prev, nb, hipMemcpyHostToDevice, h->s));
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(5 * n, 256)), dim3(256), 0, h->s, (long)n,
+                           (const double *)h->stage, h->Pprev);
+        HIPCHK(hipGetLastError());
+        h->have_prev = true;
+    }
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->n = n;
+    h->have_P = false;
+    return 0;
+}
+
+int vrn_c2p_solve(vrn_c2p *h, int guess, int n_iter, double rtol, vrn_c2p_stats *stats)
+{
+    if (!h) return fail(-1, "vrn_c2p_solve: NULL handle");
+    if (h->n < 1) return fail(-1, "vrn_c2p_solve: nothing uploaded");
+    const int record = (guess & RECORD_FLAG) ? 1 : 0;
+    guess &= ~RECORD_FLAG;
+    if (guess < VRN_GUESS_GENERIC || guess > VRN_GUESS_LEARNED) return fail(-1, "vrn_c2p_solve: unknown guess %d", guess);
+    if (guess == VRN_GUESS_WARM && !h->have_prev) return fail(-1, "vrn_c2p_solve: warm guess needs P_prev in vrn_c2p_upload");
+    if (guess == VRN_GUESS_LEARNED && !h->have_params) return fail(-1, "vrn_c2p_solve: learned guess needs trained parameters (vrn_mlp_set / vrn_mlp_train)");
+    if (n_iter < 1 || n_iter > 10000) return fail(-1, "vrn_c2p_solve: n_iter %d out of range", n_iter);
+    if (!(rtol > 0.0 && rtol < 1.0)) return fail(-1, "vrn_c2p_solve: rtol %g out of range", rtol);
+    SolveArgs a;
+    a.n = h->n;
+    a.U = h->U;
+    a.Pprev = h->Pprev;
+    a.P = h->P;
+    a.W0 = h->W0;
+    a.iters = h->iters;
+    a.cnt = h->cnt;
+    a.rescue = h->rescue;
+    a.g = h->g;
+    a.rtol = rtol;
+    a.n_iter = n_iter;
+    a.record = record;
+    a.params = h->params;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(128), 0, h->s, h->cnt);
+    const long nchunks = (h->n + 63) / 64;
+    const int grid = grid_for(nchunks, 4, 256 * 16);
+    HIPCHK(hipEventRecord(h->e0, h->s));
+    const int rgrid = 1024;
+    switch (guess) {
+    case VRN_GUESS_GENERIC:
+        hipLaunchKernelGGL(k_solve<VRN_GUESS_GENERIC>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_rescue<VRN_GUESS_GENERIC>, dim3(rgrid), dim3(256), 0, h->s, a);
+        break;
+    case VRN_GUESS_WARM:
+        hipLaunchKernelGGL(k_solve<VRN_GUESS_WARM>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_rescue<VRN_GUESS_WARM>, dim3(rgrid), dim3(256), 0, h->s, a);
+        break;
+    case VRN_GUESS_ANALYTIC:
+        hipLaunchKernelGGL(k_solve<VRN_GUESS_ANALYTIC>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_rescue<VRN_GUESS_ANALYTIC>, dim3(rgrid), dim3(256), 0, h->s, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_solve<VRN_GUESS_LEARNED>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_rescue<VRN_GUESS_LEARNED>, dim3(rgrid), dim3(256), 0, h->s, a);
+        break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->e1, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->have_P = true;
+    h->recorded = record;
+    if (stats) {
+        Counters c;
+        HIPCHK(hipMemcpy(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
+        stats->n = h->n;
+        stats->n_unrecovered = (int64_t)c.n_unrecovered;
+        stats->n_fallback = (int64_t)c.n_fallback;
+        stats->n_guess_bad = (int64_t)c.n_guess_bad;
+        stats->iters_sum = (int64_t)c.iters_sum;
+        stats->iters_max = (int64_t)c.iters_max;
+        stats->wave_max_sum = (int64_t)c.wave_max_sum;
+        stats->n_waves = (int64_t)c.n_waves;
+        for (int b = 0; b < 64; b++) stats->hist[b] = (int64_t)c.hist[b];
+        stats->ms = ms;
+    }
+    return 0;
+}
+
+int vrn_c2p_download(vrn_c2p *h, double *P, double *W0, uint8_t *iters)
+{
+    if (!h) return fail(-1, "vrn_c2p_download: NULL handle");
+    if (!h->have_P) return fail(-1, "vrn_c2p_download: no solve since the last upload");
+    if ((W0 || iters) && !h->recorded)
+        return fail(-1, "vrn_c2p_download: W0/iters need a solve with the record flag (guess | 0x100)");
+    const long n = h->n;
+    if (P) {
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(5 * n, 256)), dim3(256), 0, h->s, n, (const double *)h->P,
+                           h->stage);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(P, h->stage, sizeof(double) * 5 * (size_t)n, hipMemcpyDeviceToHost, h->s));
+    }
+    if (W0) HIPCHK(hipMemcpyAsync(W0, h->W0, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->s));
+    if (iters) HIPCHK(hipMemcpyAsync(iters, h->iters, (size_t)n, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int vrn_mlp_set(vrn_c2p *h, const float *params)
+{
+    if (!h || !params) return fail(-1, "vrn_mlp_set: NULL argument");
+    for (int i = 0; i < NPARAM; i++)
+        if (!std::isfinite(params[i])) return fail(-1, "vrn_mlp_set: parameter %d is not finite", i);
+    HIPCHK(hipMemcpyAsync(h->params, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->have_params = true;
+    return 0;
+}
+
+int vrn_mlp_get(vrn_c2p *h, float *params)
+{
+    if (!h || !params) return fail(-1, "vrn_mlp_get: NULL argument");
+    if (!h->have_params) return fail(-1, "vrn_mlp_get: no parameters set or trained");
+    HIPCHK(hipMemcpyAsync(params, h->params, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int vrn_mlp_features(vrn_c2p *h, int64_t n, const double *U, const double *P, float *X, float *T,
+                     uint8_t *valid)
+{
+    if (!h || !U || !X || !valid || (P && !T)) return fail(-1, "vrn_mlp_features: NULL argument");
+    if (n < 1 || n > h->n_max) return fail(-1, "vrn_mlp_features: n = %lld out of range", (long long)n);
+    if (int rc = ensure_pairs(h, n)) return rc;
+    HIPCHK(hipMemcpyAsync(h->pairU, U, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice, h->s));
+    if (P) HIPCHK(hipMemcpyAsync(h->pairP, P, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice, h->s));
+    hipLaunchKernelGGL(k_features, dim3(grid_for(n, 256)), dim3(256), 0, h->s, (long)n, (const double *)h->pairU,
+                       P ? (const double *)h->pairP : (const double *)nullptr, h->g, h->X, h->T, h->valid);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(X, h->X, sizeof(float) * NIN * (size_t)n, hipMemcpyDeviceToHost, h->s));
+    if (P) HIPCHK(hipMemcpyAsync(T, h->T, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(valid, h->valid, (size_t)n, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int vrn_mlp_infer(vrn_c2p *h, int64_t n, const float *X, float *Y)
+{
+    if (!h || !X || !Y) return fail(-1, "vrn_mlp_infer: NULL argument");
+    if (n < 1 || n > h->n_max) return fail(-1, "vrn_mlp_infer: n = %lld out of range", (long long)n);
+    if (!h->have_params) return fail(-1, "vrn_mlp_infer: no parameters set or trained");
+    if (int rc = ensure_pairs(h, n)) return rc;
+    HIPCHK(hipMemcpyAsync(h->X, X, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, h->s));
+    const long nchunks = (n + 63) / 64;
+    hipLaunchKernelGGL(k_infer, dim3(grid_for(nchunks, 4, 4096)), dim3(256), 0, h->s, (long)n, (const float *)h->X,
+                       (const float *)h->params, h->T);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Y, h->T, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int vrn_mlp_grad(vrn_c2p *h, int64_t n, const float *X, const float *T, const float *params, int qat,
+                 float *loss, float *grad)
+{
+    if (!h || !X || !T || !params || !grad) return fail(-1, "vrn_mlp_grad: NULL argument");
+    if (n < 1 || n > MAX_BATCH) return fail(-1, "vrn_mlp_grad: n = %lld outside [1, %d]", (long long)n, MAX_BATCH);
+    if (int rc = ensure_train(h)) return rc;
+    if (int rc = ensure_pairs(h, n)) return rc;
+    HIPCHK(hipMemcpyAsync(h->X, X, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->T, T, sizeof(float) * 2 * (size_t)n, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->tprm, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, h->s));
+    if (int rc = grad_pass(h, h->X, h->T, nullptr, n, (int)n, 0, 0, h->tprm, 1.0f, qat ? 1 : 0)) return rc;
+    std::vector<float> g(NPARAM + 1);
+    HIPCHK(hipMemcpyAsync(g.data(), h->grad, sizeof(float) * (NPARAM + 1), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    std::memcpy(grad, g.data(), sizeof(float) * NPARAM);
+    if (loss) *loss = g[NPARAM];
+    return 0;
+}
+
+int vrn_mlp_train(vrn_c2p *h, int64_t n, const double *U, const double *P, const float *init,
+                  const vrn_mlp_train_params *tp, float *loss_curve, double *ms)
+{
+    if (!h || !U || !P || !tp) return fail(-1, "vrn_mlp_train: NULL argument");
+    if (n < 1 || n > (int64_t)1 << 30) return fail(-1, "vrn_mlp_train: n = %lld out of range", (long long)n);
+    if (tp->steps < 1 || tp->batch < 1 || tp->batch > MAX_BATCH)
+        return fail(-1, "vrn_mlp_train: steps %d / batch %d out of range", tp->steps, tp->batch);
+    if (int rc = ensure_train(h)) return rc;
+    if (int rc = ensure_pairs(h, n)) return rc;
+    HIPCHK(hipEventRecord(h->e0, h->s));
+    HIPCHK(hipMemcpyAsync(h->pairU, U, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->pairP, P, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice, h->s));
+    hipLaunchKernelGGL(k_features, dim3(grid_for(n, 256)), dim3(256), 0, h->s, (long)n, (const double *)h->pairU,
+                       (const double *)h->pairP, h->g, h->X, h->T, h->valid);
+    HIPCHK(hipGetLastError());
+    // list of the valid pairs (host side: a one-off O(n) pass)
+    std::vector<unsigned char> ok((size_t)n);
+    HIPCHK(hipMemcpyAsync(ok.data(), h->valid, (size_t)n, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    std::vector<int> ids;
+    ids.reserve((size_t)n);
+    for (long i = 0; i < n; i++)
+        if (ok[i]) ids.push_back((int)i);
+    if (ids.empty()) return fail(-1, "vrn_mlp_train: no pair lies in the model's domain");
+    HIPCHK(hipMemcpyAsync(h->idx, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, h->s));
+    std::vector<float> w0;
+    if (init) {
+        w0.assign(init, init + NPARAM);
+    } else {
+        he_init(w0, tp->seed);
+    }
+    HIPCHK(hipMemcpyAsync(h->params, w0.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemsetAsync(h->m, 0, sizeof(float) * NPARAM, h->s));
+    HIPCHK(hipMemsetAsync(h->v, 0, sizeof(float) * NPARAM, h->s));
+    float *curve = nullptr;
+    HIPCHK(hipMalloc(&curve, sizeof(float) * (size_t)tp->steps));
+    double bc1 = 1.0, bc2 = 1.0;
+    int rc = 0;
+    for (int s = 0; s < tp->steps && rc == 0; s++) {
+        const double lr = tp->lr_min + 0.5 * (tp->lr - tp->lr_min) * (1.0 + std::cos(3.141592653589793 * s / tp->steps));
+        bc1 *= tp->beta1;
+        bc2 *= tp->beta2;
+        rc = grad_pass(h, h->X, h->T, h->idx, (long)ids.size(), tp->batch, tp->seed, s, h->params, tp->w_out1,
+                       tp->qat);
+        if (rc) break;
+        hipLaunchKernelGGL(k_adam, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, h->s, h->params, h->m, h->v,
+                           (const float *)h->grad, (float)lr, tp->beta1, tp->beta2, tp->eps, (float)(1.0 - bc1),
+                           (float)(1.0 - bc2), curve + s);
+        if (hipGetLastError() != hipSuccess) rc = fail(-2, "vrn_mlp_train: launch failed at step %d", s);
+    }
+    if (rc == 0 && loss_curve)
+        rc = hipMemcpyAsync(loss_curve, curve, sizeof(float) * (size_t)tp->steps, hipMemcpyDeviceToHost, h->s) ==
+                     hipSuccess
+                 ? 0
+                 : fail(-2, "vrn_mlp_train: loss copy failed");
+    if (hipEventRecord(h->e1, h->s) != hipSuccess || hipStreamSynchronize(h->s) != hipSuccess)
+        rc = rc ? rc : fail(-2, "vrn_mlp_train: stream error");
+    (void)hipFree(curve);
+    if (rc) return rc;
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, h->e0, h->e1));
+    if (ms) *ms = t;
+    std::vector<float> w(NPARAM);
+    HIPCHK(hipMemcpy(w.data(), h->params, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    for (int i = 0; i < NPARAM; i++)
+        if (!std::isfinite(w[i])) {
+            h->have_params = false;
+            return fail(-3, "vrn_mlp_train: training diverged (parameter %d not finite)", i);
+        }
+    h->have_params = true;
+    return 0;
+}
+
+}  // extern "C"
