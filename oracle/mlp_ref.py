@@ -8,10 +8,10 @@ W3[2][64], b3[2], row = output unit.
 Loss: mean over samples of (y0 - t0)^2 + w1 (y1 - t1)^2.
 
 `qat=True` evaluates the network the way the BF16 inference kernel does:
-inputs, W1, W2, W3 and the first hidden layer rounded to BF16 (round to
-nearest even), biases, the second hidden layer and all sums in full
-precision; the backward pass then treats the rounding as the identity
-(straight-through), with the rounded values in every product.
+inputs, W1, b1 (folded into layer 1 as the weight of a constant input), W2,
+W3 and both hidden layers rounded to BF16 (round to nearest even); b2, b3
+and all sums in full precision.  The backward pass treats the rounding as the
+identity (straight-through), with the rounded values in every product.
 
 How it is pinned: `loss_grad` against central finite differences of
 `forward` in float64 (tests/test_c2p_oracle.py), which a dropped term, a
@@ -56,8 +56,8 @@ def _q(a, qat):
 
 def forward(params, X, qat=False, dtype=np.float64):
     p = unpack(params)
-    W1, W2, W3 = (_q(p[k], qat).astype(dtype) for k in ("W1", "W2", "W3"))
-    b1, b2, b3 = (p[k].astype(dtype) for k in ("b1", "b2", "b3"))
+    W1, W2, W3, b1 = (_q(p[k], qat).astype(dtype) for k in ("W1", "W2", "W3", "b1"))
+    b2, b3 = (p[k].astype(dtype) for k in ("b2", "b3"))
     x = _q(np.asarray(X, dtype=np.float32), qat).astype(dtype)
     z1 = x @ W1.T + b1
     h1 = np.maximum(z1, 0)
@@ -65,6 +65,8 @@ def forward(params, X, qat=False, dtype=np.float64):
         h1 = bf16_round(h1.astype(np.float32)).astype(dtype)
     z2 = h1 @ W2.T + b2
     h2 = np.maximum(z2, 0)
+    if qat:
+        h2 = bf16_round(h2.astype(np.float32)).astype(dtype)
     y = h2 @ W3.T + b3
     return dict(x=x, z1=z1, h1=h1, z2=z2, h2=h2, y=y, W1=W1, W2=W2, W3=W3)
 

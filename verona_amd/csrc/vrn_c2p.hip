@@ -18,12 +18,14 @@
 // log1p(|S|/tau), |S|/E and sqrt(z); none depends on the direction of S.
 //
 // Inference runs inside the solve kernel: a wave takes 64 cells, computes
-// their features, and evaluates the 5 -> 64 -> 64 layers with
-// v_mfma_f32_16x16x16_bf16 in the transposed orientation H^T = W X^T, so the
-// 16x16 accumulator tile of one layer (rows = hidden units, columns = cells)
-// is, element for element, the B operand of the next layer: no LDS round trip
-// between layers.  The 64 -> 2 output layer is 32 FP32 FMAs per lane plus a
-// 4-way cross-lane sum, so the second hidden layer is never rounded to BF16.
+// their features, and evaluates all three layers on the matrix cores in the
+// transposed orientation H^T = W X^T (layer 1 with v_mfma_f32_16x16x16_bf16,
+// its bias folded in as a constant input; layer 2 and the output layer with
+// gfx950's v_mfma_f32_16x16x32_bf16), so the accumulator tiles of one layer
+// (rows = hidden units, columns = cells) are, element for element, the B
+// operand of the next: no LDS round trip between layers.  BF16 values:
+// inputs, W1 with b1, W2, W3 and both hidden layers; FP32: accumulation,
+// b2, b3.
 //
 // Training (FP32 VALU): one thread per sample runs forward and backward with
 // the parameters in LDS and caches per-sample activations and deltas; weight
@@ -191,6 +193,11 @@ struct Counters {
     unsigned long long n_rescue;  // entries in the rescue list
 };
 
+// Per-block partial counters: row of PART_STRIDE values, columns
+// 0 unrecovered, 1 fallback, 2 guess_bad, 3 iters_sum, 4 wave_max_sum,
+// 5 n_waves, 6 iters_max, 8 + b histogram bin b.
+constexpr int PART_STRIDE = 72;
+
 struct Rescue {
     long i;     // cell
     double W0;  // its initial W
@@ -204,6 +211,7 @@ struct SolveArgs {
     double *W0;           // per-cell initial W (record mode)
     unsigned char *iters; // per-cell iterations (record mode)
     Counters *cnt;
+    unsigned long long *part;  // [gridDim.x][PART_STRIDE]
     Rescue *rescue;       // cells the 1DW iteration did not finish
     double g, rtol;
     int n_iter;
@@ -211,10 +219,60 @@ struct SolveArgs {
     const float *params;  // FP32 parameters (learned guess)
 };
 
-// Weights of the learned guess as one wave holds them.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// Two floats -> two BF16 (round to nearest even) in one v_cvt_pk_bf16_f32:
+// lo = first operand, hi = second.
+__device__ __forceinline__ unsigned cvt_pk_bf16(float lo, float hi)
+{
+    unsigned r;
+    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(lo), "v"(hi));
+    return r;
+}
+// ReLU of two packed BF16: negative values (sign bit set) are negative as
+// int16, so a signed max with 0 clears them (v_pk_max_i16).
+__device__ __forceinline__ unsigned relu_pk_bf16(unsigned x)
+{
+    const s16x2 z = {0, 0};
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(s16x2, x), z));
+}
+// ReLU + BF16 of an accumulator tile's four values, as one 64-bit word.
+__device__ __forceinline__ unsigned long long relu_pack4(const f32x4 &v)
+{
+    const unsigned lo = relu_pk_bf16(cvt_pk_bf16(v[0], v[1]));
+    const unsigned hi = relu_pk_bf16(cvt_pk_bf16(v[2], v[3]));
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ bf16x8 cat8(unsigned long long lo, unsigned long long hi)
+{
+    const u64x2 v = {lo, hi};
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// Lane l of v_mfma_f32_16x16x16_bf16 supplies A[m = l%16][k = 4 (l/16) + j]
+// and B[k = 4 (l/16) + j][n = l%16] (j = 0..3); v_mfma_f32_16x16x32_bf16
+// the same with k = 8 (l/16) + j (j = 0..7); both leave D[m = 4 (l/16) + j]
+// [n = l%16] (j = 0..3) in lane l.  The hidden layers are evaluated as
+// H^T = W X^T (rows = hidden units, columns = cells).  For a 16x16x32 layer
+// the 32 hidden units of K block kb are ordered so that lane group g holds
+// units 32 kb + 4 g + (0..3) and 32 kb + 16 + 4 g + (0..3): exactly the
+// values lane group g received in the two 16-row accumulator tiles 2 kb and
+// 2 kb + 1 of the previous layer, so a tile pair becomes the next B operand
+// without any data movement; the A fragments (weights) use the same order.
+__device__ __forceinline__ int hidden_of(int kb, int g, int jj)
+{
+    return 32 * kb + (jj < 4 ? 4 * g + jj : 16 + 4 * g + (jj - 4));
+}
+
+// Weights of the learned guess as one wave holds them (registers).
 struct NetRegs {
-    bf16x4 w1[4];      // A fragments of W1 (16 x 16 tiles, K padded 5 -> 16)
-    bf16x4 w2[4][4];   // A fragments of W2, [row tile][k tile]
+    bf16x4 w1[4];      // layer 1, K = 16: inputs x0..x4, the constant 1 (bias b1), zeros
+    bf16x8 w2[4][2];   // layer 2 [row tile][K block]
+    bf16x8 w3[2];      // output layer, rows 0 and 1 = the two outputs [K block]
+    f32x4 b2[4];       // layer 2 bias, accumulator initialisation
+    f32x4 b3;          // output bias (rows 0, 1 live in lane group 0)
 };
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
@@ -227,108 +285,88 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
     return r;
 }
 
-// Lane l of a 16x16x16 MFMA supplies A[m = l % 16][k = 4 (l / 16) + j] and
-// B[k = 4 (l / 16) + j][n = l % 16] (j = 0..3) and holds
-// D[m = 4 (l / 16) + j][n = l % 16].
+__device__ __forceinline__ bf16x8 frag8(const unsigned short *h)
+{
+    typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+    u16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = h[j];
+    return __builtin_bit_cast(bf16x8, v);
+}
+
 __device__ __forceinline__ void load_net(const float *prm, NetRegs &nr, int lane)
 {
-    const int m = lane & 15, kq = 4 * (lane >> 4);
+    const int m = lane & 15, g = lane >> 4, kq = 4 * g;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         float v[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int k = kq + j;
-            v[j] = k < NIN ? prm[OFF_W1 + (16 * r + m) * NIN + k] : 0.0f;
+            v[j] = k < NIN ? prm[OFF_W1 + (16 * r + m) * NIN + k] : (k == NIN ? prm[OFF_B1 + 16 * r + m] : 0.0f);
         }
         nr.w1[r] = pack4(v[0], v[1], v[2], v[3]);
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) {
-            const float *w = prm + OFF_W2 + (16 * r + m) * NH + 16 * kt + kq;
-            nr.w2[r][kt] = pack4(w[0], w[1], w[2], w[3]);
+        for (int kb = 0; kb < 2; kb++) {
+            unsigned short h[8];
+#pragma unroll
+            for (int jj = 0; jj < 8; jj++) h[jj] = bf16_bits(prm[OFF_W2 + (16 * r + m) * NH + hidden_of(kb, g, jj)]);
+            nr.w2[r][kb] = frag8(h);
         }
+#pragma unroll
+        for (int j = 0; j < 4; j++) nr.b2[r][j] = prm[OFF_B2 + 16 * r + kq + j];
     }
-}
-
-// Shared copy of the small per-unit parameters (b1, b2, BF16-rounded W3, b3).
-struct NetShared {
-    float b1[NH], b2[NH], w3[NOUT][NH], b3[NOUT];
-};
-
-__device__ __forceinline__ void load_net_shared(const float *prm, NetShared &ns)
-{
-    for (int i = threadIdx.x; i < NH; i += blockDim.x) {
-        ns.b1[i] = prm[OFF_B1 + i];
-        ns.b2[i] = prm[OFF_B2 + i];
-        ns.w3[0][i] = bf16_round(prm[OFF_W3 + i]);
-        ns.w3[1][i] = bf16_round(prm[OFF_W3 + NH + i]);
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) {
+        unsigned short h[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) h[jj] = m < NOUT ? bf16_bits(prm[OFF_W3 + m * NH + hidden_of(kb, g, jj)]) : 0;
+        nr.w3[kb] = frag8(h);
     }
-    if (threadIdx.x < NOUT) ns.b3[threadIdx.x] = prm[OFF_B3 + threadIdx.x];
+#pragma unroll
+    for (int j = 0; j < 4; j++) nr.b3[j] = (g == 0 && j < NOUT) ? prm[OFF_B3 + j] : 0.0f;
 }
 
 // Network outputs for the 64 cells of a wave; lane c passes the features of
-// cell c and receives that cell's outputs.  All 64 lanes must be active.
-__device__ __forceinline__ void net_eval(const NetRegs &nr, const NetShared &ns, const float *x,
-                                         int lane, float &y0, float &y1)
+// cell c and receives that cell's outputs (y1 only when BOTH).  All 64 lanes
+// must be active.
+template <bool BOTH>
+__device__ __forceinline__ void net_eval(const NetRegs &nr, const float *x, int lane, float &y0, float &y1)
 {
-    const int q = lane >> 4, kq = 4 * q;
-    const unsigned p01 = (unsigned)bf16_bits(x[0]) | ((unsigned)bf16_bits(x[1]) << 16);
-    const unsigned p23 = (unsigned)bf16_bits(x[2]) | ((unsigned)bf16_bits(x[3]) << 16);
-    const unsigned p4 = (unsigned)bf16_bits(x[4]);
+    const int g = lane >> 4;
+    const unsigned p01 = cvt_pk_bf16(x[0], x[1]);
+    const unsigned p23 = cvt_pk_bf16(x[2], x[3]);
+    const unsigned p45 = cvt_pk_bf16(x[4], 1.0f);  // x4 and the constant input of the folded bias
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int src = 16 * t + (lane & 15);
         const unsigned a01 = (unsigned)__shfl((int)p01, src);
         const unsigned a23 = (unsigned)__shfl((int)p23, src);
-        const unsigned a4 = (unsigned)__shfl((int)p4, src);
-        bf16x4 bx;
-        if (q == 0) {
-            bx[0] = (short)(a01 & 0xffff);
-            bx[1] = (short)(a01 >> 16);
-            bx[2] = (short)(a23 & 0xffff);
-            bx[3] = (short)(a23 >> 16);
-        } else {
-            bx[0] = (short)(q == 1 ? a4 : 0u);
-            bx[1] = 0;
-            bx[2] = 0;
-            bx[3] = 0;
-        }
-        bf16x4 h1[4];
+        const unsigned a45 = (unsigned)__shfl((int)p45, src);
+        const unsigned lo = g == 0 ? a01 : (g == 1 ? a45 : 0u);
+        const unsigned hi = g == 0 ? a23 : 0u;
+        const bf16x4 bx = __builtin_bit_cast(bf16x4, ((unsigned long long)hi << 32) | lo);
+        unsigned long long h1[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            f32x4 acc;
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[j] = ns.b1[16 * r + kq + j];
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w1[r], bx, acc, 0, 0, 0);
-            h1[r] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f),
-                          fmaxf(acc[3], 0.0f));
-        }
-        float s0 = 0.0f, s1 = 0.0f;
+        for (int r = 0; r < 4; r++)
+            h1[r] = relu_pack4(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w1[r], bx, zero, 0, 0, 0));
+        const bf16x8 b1a = cat8(h1[0], h1[1]), b1b = cat8(h1[2], h1[3]);
+        unsigned long long h2[4];
 #pragma unroll
         for (int r2 = 0; r2 < 4; r2++) {
-            f32x4 acc;
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[j] = ns.b2[16 * r2 + kq + j];
-#pragma unroll
-            for (int kt = 0; kt < 4; kt++)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w2[r2][kt], h1[kt], acc, 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float h = fmaxf(acc[j], 0.0f);
-                s0 = s0 + ns.w3[0][16 * r2 + kq + j] * h;
-                s1 = s1 + ns.w3[1][16 * r2 + kq + j] * h;
-            }
+            f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w2[r2][0], b1a, nr.b2[r2], 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w2[r2][1], b1b, acc, 0, 0, 0);
+            h2[r2] = relu_pack4(acc);
         }
-        // sum the four row groups: lanes l, l^16, l^32, l^48 (same cell)
-        s0 = s0 + __shfl_xor(s0, 16);
-        s1 = s1 + __shfl_xor(s1, 16);
-        s0 = s0 + __shfl_xor(s0, 32);
-        s1 = s1 + __shfl_xor(s1, 32);
-        // lane l now holds the outputs of cell 16 t + (l & 15); it is its own
-        // cell when t == l / 16
-        if (t == q) {
-            y0 = s0 + ns.b3[0];
-            y1 = s1 + ns.b3[1];
+        f32x4 out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w3[0], cat8(h2[0], h2[1]), nr.b3, 0, 0, 0);
+        out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w3[1], cat8(h2[2], h2[3]), out, 0, 0, 0);
+        // lanes 0..15 hold the outputs of cells 16 t + lane; deliver to their owners
+        const float v0 = __shfl(out[0], lane & 15);
+        if (t == g) y0 = v0;
+        if constexpr (BOTH) {
+            const float v1 = __shfl(out[1], lane & 15);
+            if (t == g) y1 = v1;
         }
     }
 }
@@ -337,37 +375,56 @@ __device__ __forceinline__ void net_eval(const NetRegs &nr, const NetShared &ns,
 template <int GUESS>
 __global__ void __launch_bounds__(256) k_solve(SolveArgs a)
 {
-    __shared__ NetShared ns;
     __shared__ unsigned hist[64];
+    __shared__ unsigned long long wtot[4][8];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < 64; i += blockDim.x) hist[i] = 0;
+    unsigned hcnt = 0;  // lane b < 16: cells of this wave that took b iterations
     NetRegs nr;
-    if constexpr (GUESS == VRN_GUESS_LEARNED) {
-        load_net_shared(a.params, ns);
-        load_net(a.params, nr, lane);
-    }
+    if constexpr (GUESS == VRN_GUESS_LEARNED) load_net(a.params, nr, lane);
     __syncthreads();
     const long n = a.n;
     const long nchunks = (n + 63) / 64;
     unsigned long long unrec = 0, fall = 0, bad = 0, isum = 0, wmax_sum = 0, nw = 0;
     int imax = 0;
-    for (long ch = (long)blockIdx.x * 4 + wid; ch < nchunks; ch += (long)gridDim.x * 4) {
+    const long stride = (long)gridDim.x * 4;
+    long ch = (long)blockIdx.x * 4 + wid;
+    // the next chunk's inputs are loaded while this chunk computes
+    double un[5], pn[5];
+    {
+        const long i = ch * 64 + lane;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            un[v] = (ch < nchunks && i < n) ? a.U[v * n + i] : 0.0;
+            if constexpr (GUESS == VRN_GUESS_WARM) pn[v] = (ch < nchunks && i < n) ? a.Pprev[v * n + i] : 1.0;
+        }
+    }
+    for (; ch < nchunks; ch += stride) {
         const long i = ch * 64 + lane;
         const bool act = i < n;
         double u[5];
+        double Pg[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) u[v] = act ? a.U[v * n + i] : 0.0;
+        for (int v = 0; v < 5; v++) {
+            u[v] = un[v];
+            if constexpr (GUESS == VRN_GUESS_WARM) Pg[v] = pn[v];
+        }
+        {
+            const long cn = ch + stride, in = cn * 64 + lane;
+            const bool an = cn < nchunks && in < n;
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                un[v] = an ? a.U[v * n + in] : 0.0;
+                if constexpr (GUESS == VRN_GUESS_WARM) pn[v] = an ? a.Pprev[v * n + in] : 1.0;
+            }
+        }
         const double S2 = u[2] * u[2] + u[3] * u[3] + u[4] * u[4];
         double W;
-        double Pg[5];
         if constexpr (GUESS == VRN_GUESS_GENERIC) {
             Pg[0] = 1.0;
             Pg[1] = 1.0;
             Pg[2] = Pg[3] = Pg[4] = 0.0;
-        } else if constexpr (GUESS == VRN_GUESS_WARM) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) Pg[v] = act ? a.Pprev[v * n + i] : 1.0;
         }
         if constexpr (GUESS == VRN_GUESS_GENERIC || GUESS == VRN_GUESS_WARM) {
             // W from the guess as UtoP forms it (src/3dim/structs.jl:276-287)
@@ -378,8 +435,7 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a)
         } else {
             const Feat f = c4_features(u[0], u[1], S2);
             float y0 = 0.0f, y1 = 0.0f;
-            net_eval(nr, ns, f.x, lane, y0, y1);
-            (void)y1;
+            net_eval<false>(nr, f.x, lane, y0, y1);
             if (f.ok && __builtin_isfinite(y0)) {
                 W = learned_w0(f, y0);
             } else {
@@ -413,30 +469,82 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs a)
             }
             isum += it;
             imax = it > imax ? it : imax;
-            atomicAdd(&hist[it < 63 ? it : 63], 1u);
+            if (it >= 16) atomicAdd(&hist[it < 63 ? it : 63], 1u);
         }
-        const int wm = wave_max_i(act ? it : 0);
-        if (lane == 0) {
-            wmax_sum += wm;
-            nw++;
+        // histogram of the common counts by ballot: lane b keeps bin b
+        const int mine = act ? it : 0;
+#pragma unroll
+        for (int b = 1; b < 16; b++) {
+            const unsigned c = (unsigned)__popcll(__ballot(mine == b));
+            hcnt += lane == b ? c : 0u;
         }
+        const int wm = wave_max_i(mine);
+        wmax_sum += wm;
+        nw++;
     }
-    // per-wave totals, then one atomic per wave
+    // per-wave totals -> block row (no global atomics on shared addresses)
     const long long t_unrec = wave_sum_ll((long long)unrec), t_fall = wave_sum_ll((long long)fall),
                     t_bad = wave_sum_ll((long long)bad), t_isum = wave_sum_ll((long long)isum);
     const int t_imax = wave_max_i(imax);
+    if (lane >= 1 && lane < 16 && hcnt) atomicAdd(&hist[lane], hcnt);
     if (lane == 0) {
-        if (t_unrec) atomicAdd(&a.cnt->n_unrecovered, (unsigned long long)t_unrec);
-        if (t_fall) atomicAdd(&a.cnt->n_fallback, (unsigned long long)t_fall);
-        if (t_bad) atomicAdd(&a.cnt->n_guess_bad, (unsigned long long)t_bad);
-        atomicAdd(&a.cnt->iters_sum, (unsigned long long)t_isum);
-        atomicAdd(&a.cnt->wave_max_sum, wmax_sum);
-        atomicAdd(&a.cnt->n_waves, nw);
-        atomicMax(&a.cnt->iters_max, (unsigned long long)t_imax);
+        wtot[wid][0] = (unsigned long long)t_unrec;
+        wtot[wid][1] = (unsigned long long)t_fall;
+        wtot[wid][2] = (unsigned long long)t_bad;
+        wtot[wid][3] = (unsigned long long)t_isum;
+        wtot[wid][4] = wmax_sum;
+        wtot[wid][5] = nw;
+        wtot[wid][6] = (unsigned long long)t_imax;
+        wtot[wid][7] = 0;
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < 64; b += blockDim.x)
-        if (hist[b]) atomicAdd(&a.cnt->hist[b], (unsigned long long)hist[b]);
+    unsigned long long *row = a.part + (long)blockIdx.x * PART_STRIDE;
+    if (threadIdx.x < 8) {
+        const int c = threadIdx.x;
+        const int nwv = (int)(blockDim.x >> 6);
+        unsigned long long v = wtot[0][c];
+        for (int w = 1; w < nwv; w++) v = c == 6 ? (wtot[w][c] > v ? wtot[w][c] : v) : v + wtot[w][c];
+        row[c] = v;
+    } else if (threadIdx.x >= 64 && threadIdx.x < 128) {
+        row[8 + threadIdx.x - 64] = hist[threadIdx.x - 64];
+    }
+}
+
+// Sum of the per-block rows (column max for iters_max): one block per column.
+__global__ void __launch_bounds__(256) k_reduce_counters(const unsigned long long *__restrict__ part, int rows,
+                                                         Counters *cnt)
+{
+    __shared__ unsigned long long red[256];
+    const int c = blockIdx.x;
+    if (c == 7) return;
+    const bool is_max = c == 6;
+    unsigned long long v = 0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        const unsigned long long x = part[(long)r * PART_STRIDE + c];
+        v = is_max ? (x > v ? x : v) : v + x;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const unsigned long long y = red[threadIdx.x + o];
+            red[threadIdx.x] = is_max ? (y > red[threadIdx.x] ? y : red[threadIdx.x]) : red[threadIdx.x] + y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned long long t = red[0];
+        switch (c) {
+        case 0: cnt->n_unrecovered = t; break;
+        case 1: cnt->n_fallback = t; break;
+        case 2: cnt->n_guess_bad = t; break;
+        case 3: cnt->iters_sum = t; break;
+        case 4: cnt->wave_max_sum = t; break;
+        case 5: cnt->n_waves = t; break;
+        case 6: cnt->iters_max = t; break;
+        default: cnt->hist[c - 8] = t; break;
+        }
+    }
 }
 
 // Cells the 1DW iteration left unfinished: the rest of the reference cascade
@@ -523,12 +631,9 @@ __global__ void k_features(long n, const double *__restrict__ U, const double *_
 __global__ void __launch_bounds__(256) k_infer(long n, const float *__restrict__ X, const float *prm,
                                                float *__restrict__ Y)
 {
-    __shared__ NetShared ns;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     NetRegs nr;
-    load_net_shared(prm, ns);
     load_net(prm, nr, lane);
-    __syncthreads();
     const long nchunks = (n + 63) / 64;
     for (long ch = (long)blockIdx.x * 4 + wid; ch < nchunks; ch += (long)gridDim.x * 4) {
         const long i = ch * 64 + lane;
@@ -536,7 +641,7 @@ __global__ void __launch_bounds__(256) k_infer(long n, const float *__restrict__
 #pragma unroll
         for (int k = 0; k < NIN; k++) x[k] = i < n ? X[NIN * i + k] : 0.0f;
         float y0 = 0.0f, y1 = 0.0f;
-        net_eval(nr, ns, x, lane, y0, y1);
+        net_eval<true>(nr, x, lane, y0, y1);
         if (i < n) {
             Y[2 * i] = y0;
             Y[2 * i + 1] = y1;
@@ -576,7 +681,8 @@ __global__ void __launch_bounds__(256) k_train_fwd_bwd(TrainArgs a)
     __shared__ float w[NPARAM];
     for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) {
         float v = a.prm[p];
-        const bool is_w = p < OFF_B1 || (p >= OFF_W2 && p < OFF_B2) || (p >= OFF_W3 && p < OFF_B3);
+        // BF16 in inference: W1 with b1 (folded into layer 1), W2, W3
+        const bool is_w = p < OFF_B2 || (p >= OFF_W3 && p < OFF_B3);
         w[p] = (a.qat && is_w) ? bf16_round(v) : v;
     }
     __syncthreads();
@@ -611,7 +717,8 @@ __global__ void __launch_bounds__(256) k_train_fwd_bwd(TrainArgs a)
         float z = w[OFF_B2 + j];
 #pragma unroll
         for (int k = 0; k < NH; k++) z = z + w[OFF_W2 + j * NH + k] * h1[k];
-        const float h = fmaxf(z, 0.0f);
+        float h = fmaxf(z, 0.0f);
+        if (a.qat) h = bf16_round(h);
         c[C_H2 + j] = h;
         y0 = y0 + w[OFF_W3 + j] * h;
         y1 = y1 + w[OFF_W3 + NH + j] * h;
@@ -721,6 +828,7 @@ int grid_for(long work, int block, int cap = 8192)
 }
 
 constexpr int MAX_BATCH = 65536;
+constexpr int MAX_SOLVE_GRID = 4096;
 constexpr int NSPLIT = 64;
 }  // namespace
 
@@ -733,6 +841,7 @@ struct vrn_c2p {
     unsigned char *iters = nullptr;
     Counters *cnt = nullptr;
     Rescue *rescue = nullptr;
+    unsigned long long *crow = nullptr;  // per-block counter rows
     float *params = nullptr;  // device FP32 parameters
     bool have_params = false, have_prev = false, have_P = false, recorded = false;
     // training / test buffers (allocated on first use)
@@ -857,6 +966,7 @@ int vrn_c2p_create(int64_t n_max, double gamma, vrn_c2p **out)
         hipMalloc(&h->stage, nb) != hipSuccess || hipMalloc(&h->W0, sizeof(double) * n_max) != hipSuccess ||
         hipMalloc(&h->iters, n_max) != hipSuccess || hipMalloc(&h->cnt, sizeof(Counters)) != hipSuccess ||
         hipMalloc(&h->rescue, sizeof(Rescue) * n_max) != hipSuccess ||
+        hipMalloc(&h->crow, sizeof(unsigned long long) * PART_STRIDE * MAX_SOLVE_GRID) != hipSuccess ||
         hipMalloc(&h->params, sizeof(float) * NPARAM) != hipSuccess)
         return cleanup(fail(-2, "vrn_c2p_create: device allocation of %zu bytes failed", 4 * nb));
     *out = h;
@@ -875,6 +985,7 @@ int vrn_c2p_destroy(vrn_c2p *h)
     (void)hipFree(h->iters);
     (void)hipFree(h->cnt);
     (void)hipFree(h->rescue);
+    (void)hipFree(h->crow);
     (void)hipFree(h->params);
     (void)hipFree(h->cache);
     (void)hipFree(h->part);
@@ -940,6 +1051,7 @@ int vrn_c2p_solve(vrn_c2p *h, int guess, int n_iter, double rtol, vrn_c2p_stats 
     a.iters = h->iters;
     a.cnt = h->cnt;
     a.rescue = h->rescue;
+    a.part = h->crow;
     a.g = h->g;
     a.rtol = rtol;
     a.n_iter = n_iter;
@@ -947,24 +1059,32 @@ int vrn_c2p_solve(vrn_c2p *h, int guess, int n_iter, double rtol, vrn_c2p_stats 
     a.params = h->params;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(128), 0, h->s, h->cnt);
     const long nchunks = (h->n + 63) / 64;
-    const int grid = grid_for(nchunks, 4, 256 * 16);
+    const int grid = grid_for(nchunks, 4, MAX_SOLVE_GRID);
     HIPCHK(hipEventRecord(h->e0, h->s));
     const int rgrid = 1024;
     switch (guess) {
     case VRN_GUESS_GENERIC:
         hipLaunchKernelGGL(k_solve<VRN_GUESS_GENERIC>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_reduce_counters, dim3(PART_STRIDE), dim3(256), 0, h->s,
+                           (const unsigned long long *)h->crow, grid, h->cnt);
         hipLaunchKernelGGL(k_rescue<VRN_GUESS_GENERIC>, dim3(rgrid), dim3(256), 0, h->s, a);
         break;
     case VRN_GUESS_WARM:
         hipLaunchKernelGGL(k_solve<VRN_GUESS_WARM>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_reduce_counters, dim3(PART_STRIDE), dim3(256), 0, h->s,
+                           (const unsigned long long *)h->crow, grid, h->cnt);
         hipLaunchKernelGGL(k_rescue<VRN_GUESS_WARM>, dim3(rgrid), dim3(256), 0, h->s, a);
         break;
     case VRN_GUESS_ANALYTIC:
         hipLaunchKernelGGL(k_solve<VRN_GUESS_ANALYTIC>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_reduce_counters, dim3(PART_STRIDE), dim3(256), 0, h->s,
+                           (const unsigned long long *)h->crow, grid, h->cnt);
         hipLaunchKernelGGL(k_rescue<VRN_GUESS_ANALYTIC>, dim3(rgrid), dim3(256), 0, h->s, a);
         break;
     default:
         hipLaunchKernelGGL(k_solve<VRN_GUESS_LEARNED>, dim3(grid), dim3(256), 0, h->s, a);
+        hipLaunchKernelGGL(k_reduce_counters, dim3(PART_STRIDE), dim3(256), 0, h->s,
+                           (const unsigned long long *)h->crow, grid, h->cnt);
         hipLaunchKernelGGL(k_rescue<VRN_GUESS_LEARNED>, dim3(rgrid), dim3(256), 0, h->s, a);
         break;
     }

@@ -17,7 +17,8 @@
 //    of cells over many rows and passes the upper face state and the face
 //    flux to the next lane with a cross-lane shift; the y and z sweeps give
 //    each lane one (x, .) column and march along the axis with the 5-cell
-//    reconstruction window held in registers.  Either way each cell's
+//    reconstruction window held in an LDS ring (5 slots x NV variables per
+//    lane, so advancing the window moves no data).  Either way each cell's
 //    reconstruction and each face's Riemann solve run exactly once, and all
 //    global loads and stores are row-contiguous across the wave.  The last
 //    sweep of a stage also forms the RK combination, runs 1DW con2prim +
