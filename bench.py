@@ -95,6 +95,57 @@ def run_gpu(n, steps, warmup, *, ref, precision="fp64fast", timing=True):
     return el, rep, nbad, nresc
 
 
+def run_kh(n=2048, steps=40, warmup=3, precision="fp64fast"):
+    """2D relativistic Kelvin-Helmholtz, n^2 periodic, WENOZ + HLLC + SSP-RK3, CFL 0.4."""
+    from verona_amd import DeviceGrid
+    from verona_amd.problems import kelvin_helmholtz
+    dx = 1.0 / n
+    bc = ["periodic"] * 4 + ["none", "none"]
+    with DeviceGrid(2, n, n, 1, GAMMA) as dev:
+        dev.upload("P", kelvin_helmholtz(n, gamma=GAMMA))
+        dev.fill_ghosts(bc)
+        dev.prim_to_cons()
+        dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision=precision,
+                     n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, bc=bc)
+        for _ in range(warmup):
+            dev.step()
+        dev.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.step()
+        dev.sync()
+        el = time.perf_counter() - t0
+        nbad = dev.counters()[0]
+    return {"what": f"2D Kelvin-Helmholtz {n}^2 periodic, WENOZ+HLLC+SSP-RK3, CFL 0.4, {precision}",
+            "value": n * n * steps / el, "unit": "zone-updates/s", "steps": steps,
+            "ms_per_step": el / steps * 1e3, "n_unrecovered": nbad}
+
+
+def fp32_error(n, steps):
+    """Relative L1 per variable of the FP32 reconstruction/Riemann form against the
+    all-FP64 (exact) form after `steps` steps of the jet from the same state."""
+    out = {}
+    for prec in ("fp64", "fp32"):
+        from verona_amd import DeviceGrid
+        dx = 1.0 / n
+        nz = nozzle(n)
+        phis = phi_stream(steps)
+        with DeviceGrid(3, n, n, n, GAMMA) as dev:
+            dev.upload("P", jet_state(n))
+            dev.set_nozzle(nz["radius"], dx, dx, nz["rho"], nz["u"], nz["umag"], nz["eps"])
+            dev.fill_ghosts(BC_JET)
+            dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision=prec,
+                         n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx, bc=BC_JET)
+            dev.prim_to_cons()
+            for s in range(steps):
+                dev.step(phi=phis[s])
+            out[prec] = dev.download("P")[:, 3:-3, 3:-3, 3:-3]
+    a, b = out["fp32"], out["fp64"]
+    err = [float(np.abs(a[v] - b[v]).sum() / np.abs(b[v]).sum()) for v in range(5)]
+    return {"what": f"FP32 recon/Riemann vs all-FP64, jet {n}^3 after {steps} SSP-RK3 steps",
+            "rel_l1_per_var": dict(zip(["rho", "u", "u^x", "u^y", "u^z"], err))}
+
+
 def cpu_baseline(budget_s=15.0, n=128):
     """CPU oracle (plain-C restatement of Verona's algorithm, OpenMP) on a
     bounded sample of the same workload: an n^3 grid, SSP-RK3 steps until
@@ -284,6 +335,10 @@ def main():
         out["fp32_form"] = {"what": "reconstruction + Riemann in FP32, con2prim + accumulation FP64",
                             "value": cells * max(K // 2, 4) / el3, "unit": "zone-updates/s",
                             "n_unrecovered": nb3}
+        log("fp32 error")
+        out["fp32_form"]["error_vs_fp64"] = fp32_error(n, 10)
+        log("kelvin-helmholtz")
+        out["kelvin_helmholtz"] = run_kh()
     if not args.no_c2p:
         out["con2prim"] = run_c2p(log=log)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline()

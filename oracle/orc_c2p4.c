@@ -93,16 +93,17 @@ void orc4_features(const double *U, long n, float *X, double *E, double *z, unsi
         const double S2 = u[2] * u[2] + u[3] * u[3] + u[4] * u[4];
         const double e = -u[1];
         const double tau = e - D;
-        const double Sm = sqrt(S2);
-        const double zz = (tau * (e + D) - S2) / (e * e);
+        const double rE = 1.0 / e;
+        const double zz = (tau * (e + D) - S2) * (rE * rE);
         const int good = D > 0.0 && tau > 0.0 && zz > 0.0 && zz < 1.0 && isfinite(e) && isfinite(S2);
         float *x = X + 5 * c;
         if (good) {
             const float tf = (float)tau;
+            const float Sm = sqrtf((float)S2);
             x[0] = 0.1f * logf((float)D);
             x[1] = 0.1f * logf(tf);
-            x[2] = 0.2f * log1pf((float)Sm / tf);
-            x[3] = (float)(Sm / e);
+            x[2] = 0.2f * logf(1.0f + Sm / tf);
+            x[3] = Sm * (float)rE;
             x[4] = sqrtf((float)zz);
         } else {
             x[0] = x[1] = x[2] = x[3] = x[4] = 0.0f;

@@ -112,3 +112,40 @@ def test_fast_fp64_form2(oracle):
     errs = per_var_rel_l1(interior(Pg), interior(from_oracle2(Po)))
     progress(f"  per-var rel L1 {['%.2e' % e for e in errs]}")
     assert max(errs) <= TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_kelvin_helmholtz(oracle, precision):
+    """The 2D Kelvin-Helmholtz problem (periodic, WENOZ + HLLC + SSP-RK3, CFL
+    0.4) at 128^2: bit-level parity in FP64 after 20 steps; the FP32
+    reconstruction/Riemann form's error against it is reported and bounded."""
+    from verona_amd import DeviceGrid
+    from verona_amd.problems import kelvin_helmholtz
+    progress(f"test_kelvin_helmholtz {precision}")
+    n, steps = 128, 20
+    P = kelvin_helmholtz(n, gamma=G)
+    dx = 1.0 / n
+    bc = ["periodic"] * 4 + ["none", "none"]
+    with DeviceGrid(2, n, n, 1, G) as dev:
+        dev.upload("P", P)
+        dev.fill_ghosts(bc)
+        dev.prim_to_cons()
+        dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision=precision,
+                     n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, bc=bc)
+        for _ in range(steps):
+            dev.step()
+        Pg = dev.download("P")
+        assert dev.counters()[0] == 0
+    Po = to_oracle2(P)
+    m = oracle.make_mode(ref=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50, tol=1e-10,
+                         gamma=G, floor=1e-7, cfl=0.4, dx=dx, dy=dx)
+    bad, t, _ = oracle.run2(Po, m, [1, 1, 1, 1], steps)
+    assert bad == 0 and t > 0
+    errs = per_var_rel_l1(interior(Pg), interior(from_oracle2(Po)))
+    progress(f"  per-var rel L1 vs FP64 oracle {['%.2e' % e for e in errs]}")
+    if precision == "fp64":
+        assert max(errs) <= TOL
+    else:
+        # u^y is the 1 % transverse perturbation, so its relative error is
+        # the largest (1.0e-5 measured on MI355X)
+        assert max(errs) <= 1e-4

@@ -15,7 +15,7 @@
 // lower bound E, so the relative guess error is (error of y0) * q -- small
 // exactly where the reference's W_min clamp makes Newton slow (Gamma >> 1).
 // The second output is log Gamma.  Inputs: log D, log tau (tau = E - D),
-// log1p(|S|/tau), |S|/E and sqrt(z); none depends on the direction of S.
+// log(1 + |S|/tau), |S|/E and sqrt(z); none depends on the direction of S.
 //
 // Inference runs inside the solve kernel: a wave takes 64 cells, computes
 // their features, and evaluates all three layers on the matrix cores in the
@@ -85,16 +85,19 @@ __device__ __forceinline__ Feat c4_features(double D, double Qt, double S2)
     Feat f;
     f.E = -Qt;
     const double tau = f.E - D;
-    const double Sm = __builtin_sqrt(S2);
-    f.z = (tau * (f.E + D) - S2) / (f.E * f.E);
+    const double rE = 1.0 / f.E;
+    // E^2 - S^2 - D^2 = tau (E + D) - S^2, in FP64: it is a difference of
+    // nearly equal terms when p << E
+    f.z = (tau * (f.E + D) - S2) * (rE * rE);
     f.ok = D > 0.0 && tau > 0.0 && f.z > 0.0 && f.z < 1.0 && __builtin_isfinite(f.E) &&
            __builtin_isfinite(S2);
     if (f.ok) {
         const float tf = (float)tau;
+        const float Sm = __builtin_sqrtf((float)S2);
         f.x[0] = 0.1f * __logf((float)D);
         f.x[1] = 0.1f * __logf(tf);
-        f.x[2] = 0.2f * log1pf((float)Sm / tf);
-        f.x[3] = (float)(Sm / f.E);
+        f.x[2] = 0.2f * __logf(1.0f + __fdividef(Sm, tf));
+        f.x[3] = Sm * (float)rE;
         f.x[4] = __builtin_sqrtf((float)f.z);
     } else {
 #pragma unroll
@@ -121,7 +124,11 @@ __device__ __forceinline__ double learned_w0(const Feat &f, float y0)
 {
     float qh = (float)f.z * __expf(y0);
     qh = qh < 0.0f ? 0.0f : (qh > 0.5f ? 0.5f : qh);
-    return f.E + f.E * (double)expm1f(qh);
+    // expm1(qh): Taylor to 4th order below 1/16 (relative error < 2e-7),
+    // exp - 1 above
+    const float em = qh < 0.0625f ? qh * (1.0f + qh * (0.5f + qh * (1.0f / 6.0f + qh * (1.0f / 24.0f))))
+                                  : __expf(qh) - 1.0f;
+    return f.E + f.E * (double)em;
 }
 
 // ---------------------------------------------------------------- Newton
@@ -223,13 +230,16 @@ typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
-// Two floats -> two BF16 (round to nearest even) in one v_cvt_pk_bf16_f32:
-// lo = first operand, hi = second.
+// Two floats -> two packed BF16, round to nearest even (compiler conversion:
+// v_cvt_pk_bf16_f32 on gfx950, with the hazard wait states the compiler
+// inserts after a matrix-core result).
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned cvt_pk_bf16(float lo, float hi)
 {
-    unsigned r;
-    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(lo), "v"(hi));
-    return r;
+    const f32x2 v = {lo, hi};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2v));
 }
 // ReLU of two packed BF16: negative values (sign bit set) are negative as
 // int16, so a signed max with 0 clears them (v_pk_max_i16).
@@ -241,8 +251,9 @@ __device__ __forceinline__ unsigned relu_pk_bf16(unsigned x)
 // ReLU + BF16 of an accumulator tile's four values, as one 64-bit word.
 __device__ __forceinline__ unsigned long long relu_pack4(const f32x4 &v)
 {
-    const unsigned lo = relu_pk_bf16(cvt_pk_bf16(v[0], v[1]));
-    const unsigned hi = relu_pk_bf16(cvt_pk_bf16(v[2], v[3]));
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, __builtin_convertvector(v, bf16x4v));
+    const unsigned lo = relu_pk_bf16((unsigned)b);
+    const unsigned hi = relu_pk_bf16((unsigned)(b >> 32));
     return ((unsigned long long)hi << 32) | lo;
 }
 __device__ __forceinline__ bf16x8 cat8(unsigned long long lo, unsigned long long hi)
@@ -373,7 +384,7 @@ __device__ __forceinline__ void net_eval(const NetRegs &nr, const float *x, int 
 
 // ---------------------------------------------------------------- solve
 template <int GUESS>
-__global__ void __launch_bounds__(256) k_solve(SolveArgs a)
+__global__ void __launch_bounds__(256, 3) k_solve(SolveArgs a)
 {
     __shared__ unsigned hist[64];
     __shared__ unsigned long long wtot[4][8];

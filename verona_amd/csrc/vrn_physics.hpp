@@ -87,9 +87,28 @@ __device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T 
     const T eps = WenoEps<T>::v;
     T tau5 = dabs(beta0 - beta2);
     // beta + eps >= 1e-42, a normal double, so the fast reciprocal is safe here
+#if defined(VRN_FAST)
+    T ind0, ind1, ind2;
+    if constexpr (std::is_same<T, double>::value) {
+        // one division for the three indicators: the product of the three
+        // (beta + eps) is >= 1e-126 (normal) and far from overflow for
+        // physical data
+        const T e0 = beta0 + eps, e1 = beta1 + eps, e2 = beta2 + eps;
+        const T e01 = e0 * e1;
+        const T r = fdiv(tau5, e01 * e2);
+        ind0 = r * (e1 * e2);
+        ind1 = r * (e0 * e2);
+        ind2 = r * e01;
+    } else {
+        ind0 = fdiv(tau5, beta0 + eps);
+        ind1 = fdiv(tau5, beta1 + eps);
+        ind2 = fdiv(tau5, beta2 + eps);
+    }
+#else
     T ind0 = fdiv(tau5, beta0 + eps);
     T ind1 = fdiv(tau5, beta1 + eps);
     T ind2 = fdiv(tau5, beta2 + eps);
+#endif
     T s0, s1, s2;
     if constexpr (std::is_same<T, float>::value) {
         // FP32 form: ind^2 overflows FP32 once ind > ~1.8e19 (a flat side next
@@ -286,6 +305,32 @@ __host__ __device__ __forceinline__ void lambdas(T v, T cs, T lor2, T &lp, T &lm
 #endif
 }
 
+#if defined(VRN_FAST)
+// Fast FP64 form of one face state's v_d and wave speeds.  With
+// c^2 = g(g-1)u/(rho + g u) (eos.jl:15-17) substituted,
+//   sigma = c^2/(lor^2 (1 - c^2)) = a/(lor^2 b),  a = g(g-1)u,  b = rho + g(2-g)u,
+//   1/(1 + sigma) = d/(d + a),  d = lor^2 b,
+// so v = u_d/lor, sigma and 1/(1 + sigma) share one reciprocal of lor d (d + a)
+// and the sound speed's square root disappears.  Rounding differs from the
+// reference order at the last bits only (lor >= 1, rho and u floored > 0).
+template <int DIM>
+__device__ __forceinline__ void side_speeds(const double *P, double lor, double g, double &v, double &lp,
+                                            double &lm)
+{
+    const double a = g * (g - 1.0) * P[1];
+    const double b = P[0] + g * (2.0 - g) * P[1];
+    const double d = lor * lor * b;
+    const double s = d + a;
+    const double r = fdiv(1.0, lor * d * s);
+    v = P[2 + DIM] * (d * s * r);
+    const double sig = a * (lor * s * r);
+    const double inv = d * (lor * d * r);
+    const double rt = dsqrt(sig * (1.0 - v * v + sig));
+    lp = (v + rt) * inv;
+    lm = (v - rt) * inv;
+}
+#endif
+
 // HLL (3dim/algos.jl:158-174); HLLC with HLL fallback (Mignone & Bodo 2005,
 // no reference counterpart) written in Verona's variables (U[1] = -E, F[1] = -m_d).
 template <int NV, int RIEM, class T>
@@ -369,13 +414,21 @@ __device__ __forceinline__ void face_flux(const T *PL, const T *PR, T g, T *F)
     p2u<NV>(PL, UL, g, lorL);
     p2f<NV, DIM>(PR, FR, g, lorR);
     p2f<NV, DIM>(PL, FL, g, lorL);
-    T vL = fdiv(PL[2 + DIM], lorL);
-    T vR = fdiv(PR[2 + DIM], lorR);
-    T CL = sound_speed(PL[0], PL[1], g);
-    T CR = sound_speed(PR[0], PR[1], g);
-    T lpL, lmL, lpR, lmR;
-    lambdas(vL, CL, lorL * lorL, lpL, lmL);
-    lambdas(vR, CR, lorR * lorR, lpR, lmR);
+    T vL, vR, lpL, lmL, lpR, lmR;
+#if defined(VRN_FAST)
+    if constexpr (std::is_same<T, double>::value) {
+        side_speeds<DIM>(PL, lorL, g, vL, lpL, lmL);
+        side_speeds<DIM>(PR, lorR, g, vR, lpR, lmR);
+    } else
+#endif
+    {
+        vL = fdiv(PL[2 + DIM], lorL);
+        vR = fdiv(PR[2 + DIM], lorR);
+        T CL = sound_speed(PL[0], PL[1], g);
+        T CR = sound_speed(PR[0], PR[1], g);
+        lambdas(vL, CL, lorL * lorL, lpL, lmL);
+        lambdas(vR, CR, lorR * lorR, lpR, lmR);
+    }
     T Cmax = jmax(lpL, lpR);
     T Cmin = -jmin(lmL, lmR);
     riemann_solve<NV, RIEM>(UL, UR, FL, FR, vL, vR, (g - T(1)) * PL[1], (g - T(1)) * PR[1], Cmax,
