@@ -410,6 +410,7 @@ struct vrn_ctx {
     // tuning
     int rows_per_wave = 8;
     int seg_len = 96;
+    int xseg = 1;
     // timing
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
@@ -540,6 +541,31 @@ int check_ctx(vrn_ctx *c)
 }  // namespace
 
 // ====================================================================== C ABI
+// Work partition of the sweeps for this grid: enough waves to fill 256 CUs
+// (x sweep: whole rows per wave while rows are plentiful, else segments of
+// one row; y / z sweeps: shorter marching segments when columns are few).
+// The partition changes no arithmetic: every face is solved once and every
+// cell updated once from the same operands.
+static void partition_sweeps(vrn_ctx *c)
+{
+    const int Nxi = c->G.NX - 6, Nyi = c->G.NY - 6, Nzi = c->ndim == 3 ? c->G.NZ - 6 : 1;
+    const long target = 8192;  // waves
+    const long nrows = (long)Nyi * Nzi;
+    long rpw = nrows / target;
+    c->rows_per_wave = (int)(rpw < 1 ? 1 : (rpw > 8 ? 8 : rpw));
+    c->xseg = 1;
+    if (nrows < target) {
+        long want = (target + nrows - 1) / nrows, most = Nxi / 32;
+        c->xseg = (int)(want < most ? want : most);
+        if (c->xseg < 1) c->xseg = 1;
+        if (c->xseg > 1) c->rows_per_wave = 1;
+    }
+    const long colwaves = (long)((Nxi + 63) / 64) * (c->ndim == 3 ? (Nyi < Nzi ? Nyi : Nzi) : 1);
+    const int along = c->ndim == 3 ? (Nyi > Nzi ? Nyi : Nzi) : Nyi;
+    long seg = colwaves * along / 2048;
+    c->seg_len = (int)(seg < 24 ? 24 : (seg > 96 ? 96 : seg));
+}
+
 extern "C" {
 
 const char *vrn_last_error(void) { return g_err.c_str(); }
@@ -561,6 +587,7 @@ int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
     c->G.NZ = ndim == 3 ? nz + 6 : 1;
     c->G.px = ((XOFF + c->G.NX + 15) / 16) * 16;
     c->G.ps = (long)c->G.px * c->G.NY * c->G.NZ;
+    partition_sweeps(c);
     long cells = (long)c->G.NX * c->G.NY * c->G.NZ;
     if ((long)c->G.ps >= (1L << 31)) {
         delete c;
@@ -951,6 +978,7 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     a.dt_dev = cflm ? 1 : 0;
     a.seg_len = c->seg_len;
     a.rows_per_wave = c->rows_per_wave;
+    a.xseg = c->xseg;
     const int nst = p->integrator == VRN_INT_RK3 ? 3 : 2;
     for (int s = 0; s < nst; s++) {
         a.P = c->P;

@@ -125,9 +125,12 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const d
 }
 
 // ------------------------------------------------------------ x sweep
-// Lanes walk a stream of (row, cell) pairs: rows are interior (j, k), cells
-// c = 2 .. NX-3 hold the reconstructions; lane with cell c also solves face
-// c-1/2 (c >= 3) and updates cell c-1 (c >= 4).
+// Lanes walk a stream of (row, cell) pairs.  A wave owns either
+// rows_per_wave whole interior rows (j, k) or, when xseg > 1, one of xseg
+// segments [x0, x1) of one row.  Per row the stream holds cells
+// c = x0-1 .. x1 (their reconstructions); the lane with cell c also solves
+// face c-1/2 (c >= x0) and updates cell c-1 (c >= x0+1).  Whole rows are
+// x0 = 3, x1 = NX-3.
 template <int NV, class R, int REC, int RIEM, bool LAST>
 __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
 {
@@ -137,10 +140,22 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
     const int Nyi = G.NY - 6;
     const int Nzi = NV == 5 ? G.NZ - 6 : 1;
     const int nrows = Nyi * Nzi;
-    const int r0 = wave * a.rows_per_wave;
-    if (r0 >= nrows) return;
-    const int r1 = min(r0 + a.rows_per_wave, nrows);
-    const int E = G.NX - 4;
+    int r0, r1, x0, x1;
+    if (a.xseg > 1) {
+        const int Nxi = G.NX - 6, L = (Nxi + a.xseg - 1) / a.xseg;
+        r0 = wave / a.xseg;
+        r1 = r0 + 1;
+        x0 = 3 + (wave % a.xseg) * L;
+        x1 = min(x0 + L, G.NX - 3);
+        if (r0 >= nrows || x0 >= x1) return;
+    } else {
+        r0 = wave * a.rows_per_wave;
+        if (r0 >= nrows) return;
+        r1 = min(r0 + a.rows_per_wave, nrows);
+        x0 = 3;
+        x1 = G.NX - 3;
+    }
+    const int E = x1 - x0 + 2;
     const int total = (r1 - r0) * E;
     const long ps = G.ps;
     const R g = (R)a.g, fl = (R)a.floor_;
@@ -160,11 +175,11 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
         const bool act = s < total;
         const int ss = act ? s : total - 1;
         const int r = r0 + ss / E;
-        const int c = 2 + ss % E;
+        const int c = x0 - 1 + ss % E;
         const int j = 3 + r % Nyi;
         const int k = NV == 5 ? 3 + r / Nyi : 0;
         const long rb = gidx(G, 0, j, k);
-        const bool upd = act && c >= 4;
+        const bool upd = act && c >= x0 + 1;
         const long cu = rb + (c - 1);
         // base state of the cell this lane updates, issued before the flux work
         double ub[NV];
@@ -230,7 +245,9 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
 template <int NV, int DIM, class R, int REC, int RIEM, bool LAST>
 __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 {
-    __shared__ R win[5 * NV * 256];
+    // lanes per block: 256 in 3D (four o-rows of 64 columns), 64 in 2D
+    constexpr int WS = NV == 5 ? 256 : 64;
+    __shared__ R win[5 * NV * WS];
     const Geo G = a.geo;
     const int tid = threadIdx.x;
     const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
@@ -251,7 +268,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
     const double *__restrict__ P = a.P;
     const double *Ui = a.Uacc;
-    R *wl = win + tid;  // slot s, variable v at wl[(s * NV + v) * 256]
+    R *wl = win + tid;  // slot s, variable v at wl[(s * NV + v) * WS]
 
     // ring: cell m lives in slot (m - (m0 - 3)) mod 5
     double nx[NV];
@@ -259,7 +276,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     for (int v = 0; v < NV; v++) {
         const double *q = P + v * ps + cb;
 #pragma unroll
-        for (int k = 0; k < 4; k++) wl[(k * NV + v) * 256] = (R)q[(m0 - 3 + k) * sd];
+        for (int k = 0; k < 4; k++) wl[(k * NV + v) * WS] = (R)q[(m0 - 3 + k) * sd];
         nx[v] = q[(m0 + 1) * sd];
     }
     R upp[NV];
@@ -275,7 +292,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         const int s1 = s0 == 4 ? 0 : s0 + 1, s2 = s1 == 4 ? 0 : s1 + 1;
         const int s3 = s2 == 4 ? 0 : s2 + 1, s4 = s3 == 4 ? 0 : s3 + 1;
 #pragma unroll
-        for (int v = 0; v < NV; v++) wl[(s4 * NV + v) * 256] = (R)nx[v];
+        for (int v = 0; v < NV; v++) wl[(s4 * NV + v) * WS] = (R)nx[v];
         if (c + 3 < Nd) {
 #pragma unroll
             for (int v = 0; v < NV; v++) nx[v] = P[v * ps + cb + (c + 3) * sd];
@@ -293,8 +310,8 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         R lo[NV], up[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++)
-            recon<REC>(wl[(s0 * NV + v) * 256], wl[(s1 * NV + v) * 256], wl[(s2 * NV + v) * 256],
-                       wl[(s3 * NV + v) * 256], wl[(s4 * NV + v) * 256], lo[v], up[v]);
+            recon<REC>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
+                       wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
         lo[0] = jmax(fl, lo[0]);
         lo[1] = jmax(fl, lo[1]);
         up[0] = jmax(fl, up[0]);
@@ -320,7 +337,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
                         if constexpr (std::is_same<R, double>::value)
-                            guess[v] = wl[(s1 * NV + v) * 256];
+                            guess[v] = wl[(s1 * NV + v) * WS];
                         else
                             guess[v] = P[v * ps + cu];
                     }
@@ -376,7 +393,7 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
         SweepArgs ax = a;
         ax.dx_dim = dxs[0];
         int nrows = (G.NY - 6) * (NV == 5 ? G.NZ - 6 : 1);
-        int waves = (nrows + ax.rows_per_wave - 1) / ax.rows_per_wave;
+        int waves = ax.xseg > 1 ? nrows * ax.xseg : (nrows + ax.rows_per_wave - 1) / ax.rows_per_wave;
         timer(tctx, 0, 0);
         hipLaunchKernelGGL((k_xsweep<NV, R, REC, RIEM, false>), dim3((waves + 3) / 4), dim3(256), 0, s, ax);
         timer(tctx, 0, 1);

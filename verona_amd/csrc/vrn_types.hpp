@@ -52,8 +52,9 @@ struct SweepArgs {
     double tol;
     int n_iter;
     int do_rate;
-    int seg_len;
-    int rows_per_wave;
+    int seg_len;        // y / z sweeps: interior cells per marching segment
+    int rows_per_wave;  // x sweep: rows streamed by one wave (xseg == 1)
+    int xseg;           // x sweep: segments per row (> 1 only with rows_per_wave == 1)
 };
 
 }  // namespace vrn
