@@ -170,17 +170,19 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
         cF[v] = 0.0;
     }
     double rate = 0.0;
+    // this lane's stream position (row offset rr, cell offset cc) and row base,
+    // advanced by 64 cells per chunk without integer division
+    const int nrw = r1 - r0;
+    int rr = lane / E, cc = lane % E;
+    int jj = (r0 + rr) % Nyi;
+    long rb = gidx(G, 0, 3 + jj, NV == 5 ? 3 + (r0 + rr) / Nyi : 0);
+    const long rb_last = gidx(G, 0, 3 + (r1 - 1) % Nyi, NV == 5 ? 3 + (r1 - 1) / Nyi : 0);
     for (int s0 = 0; s0 < total; s0 += 64) {
-        const int s = s0 + lane;
-        const bool act = s < total;
-        const int ss = act ? s : total - 1;
-        const int r = r0 + ss / E;
-        const int c = x0 - 1 + ss % E;
-        const int j = 3 + r % Nyi;
-        const int k = NV == 5 ? 3 + r / Nyi : 0;
-        const long rb = gidx(G, 0, j, k);
+        const bool act = rr < nrw;
+        const int c = act ? x0 - 1 + cc : x1;
+        const long rbu = act ? rb : rb_last;
         const bool upd = act && c >= x0 + 1;
-        const long cu = rb + (c - 1);
+        const long cu = rbu + (c - 1);
         // base state of the cell this lane updates, issued before the flux work
         double ub[NV];
 #pragma unroll
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
         R lo[NV], up[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            const double *q = P + v * ps + rb + c;
+            const double *q = P + v * ps + rbu + c;
             recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
         }
         lo[0] = jmax(fl, lo[0]);
@@ -230,6 +232,17 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
         for (int v = 0; v < NV; v++) {
             cup[v] = lane63(up[v]);
             cF[v] = lane63(F[v]);
+        }
+        cc += 64;
+        while (cc >= E) {
+            cc -= E;
+            rr++;
+            if (++jj == Nyi) {
+                jj = 0;
+                rb += 7L * G.px;  // past the 6 ghost rows into the next (j, k) plane row
+            } else {
+                rb += G.px;
+            }
         }
     }
     if constexpr (LAST)
