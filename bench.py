@@ -177,15 +177,19 @@ def cpu_baseline(budget_s=15.0, n=128):
                       f"workload at {n}^3, {steps} SSP-RK3 WENOZ+HLLC steps, {el:.1f} s"}
 
 
-def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None):
+def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None, quick=False):
     """Learned-guess con2prim vs plain Newton on held-out cells of the benchmark
     law (seed 42): every guess finished by the same FP64 1DW Newton to
     |dW| < 1e-12 W.  Training pairs: the solver's own dumps (2D Kelvin-Helmholtz,
     1D blast wave) plus synthetic cells drawn with seed 7."""
     from verona_amd.c2p import Con2PrimBatch, c4_states, solver_pairs, training_set, warm_guess
     say = log or (lambda m: None)
-    say("c2p: solver dumps")
-    dumps = solver_pairs()
+    if quick:
+        # profiling sequence: short training on synthetic pairs, each guess solved twice
+        n_train, train_steps, reps, dumps = 1 << 18, 800, 1, None
+    else:
+        say("c2p: solver dumps")
+        dumps = solver_pairs()
     Ut, Pt = training_set(n_train, seed=7, gamma=GAMMA, dumps=dumps)
     say("c2p: held-out cells")
     U, P = c4_states(n_cells, seed=42, gamma=GAMMA)
@@ -197,7 +201,8 @@ def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None
     with Con2PrimBatch(n_cells, GAMMA) as cp:
         say("c2p: training")
         curve, tms = cp.train(Ut, Pt, steps=train_steps, batch=8192, lr=3e-3, seed=7)
-        out["train"] = {"pairs": int(len(Ut)), "solver_dump_pairs": int(min(len(dumps[0]), n_train // 4)),
+        out["train"] = {"pairs": int(len(Ut)),
+                        "solver_dump_pairs": int(min(len(dumps[0]), n_train // 4)) if dumps else 0,
                         "steps": train_steps, "batch": 8192, "ms": tms,
                         "loss_first": float(curve[:10].mean()), "loss_last": float(curve[-100:].mean())}
         cp.upload(U, Pprev)
@@ -211,6 +216,8 @@ def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None
             st["ms_median"] = float(np.median([r["ms"] for r in runs]))
             st["cells_per_s"] = n_cells / (st["ms_median"] * 1e-3)
             res[g] = st
+        if quick:
+            return {"guesses": res}
         Pl = cp.download()
         cp.solve("analytic")
         Pa = cp.download()
@@ -258,6 +265,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-c2p", action="store_true", help="skip the learned con2prim leg")
     ap.add_argument("--c2p-only", action="store_true", help="only the learned con2prim leg")
+    ap.add_argument("--c2p-quick", action="store_true",
+                    help="with --c2p-only: the short profiling sequence (800 training steps, 2 solves per guess)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of one 384^3 slot (known bytes) for counter calibration")
     ap.add_argument("--precision", default="fp64fast", choices=["fp64", "fp64fast"],
@@ -279,7 +288,7 @@ def main():
         return
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if args.c2p_only:
-        print(json.dumps({"con2prim": run_c2p(log=log)}), flush=True)
+        print(json.dumps({"con2prim": run_c2p(log=log, quick=args.c2p_quick)}), flush=True)
         return
     cells = n ** 3
     el, rep, nbad, nresc = run_gpu(n, K, W, ref=False, precision=args.precision)
