@@ -255,6 +255,28 @@ def measured_traffic(kernel, n, precision):
     return None if k is None else float(k["bytes"])
 
 
+def fp64_roofline(kernel, cells, mean_ms, n, precision):
+    """FP64 VALU view of the dominant kernel: FLOPs per cell (2 FMA + MUL + ADD +
+    transcendental, lane counts from the committed rocprofv3 SQ_INSTS_* pass) over its
+    event-measured launch time, against MI355X's FP64 vector peak, and the VALU issue
+    fraction (a wave64 instruction occupies a 16-lane SIMD for 4 cycles)."""
+    path = os.path.join(ROOT, "profiles", "jet384_sweeps_instruction_mix.json")
+    if n != 384 or precision != "fp64fast" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        mix = json.load(f)["per_cell_lane_instructions"].get(kernel)
+    if mix is None:
+        return None
+    flop = (2 * mix["SQ_INSTS_VALU_FMA_F64"] + mix["SQ_INSTS_VALU_MUL_F64"] + mix["SQ_INSTS_VALU_ADD_F64"]
+            + mix["SQ_INSTS_VALU_TRANS_F64"])
+    tf = flop * cells / (mean_ms * 1e-3) / 1e12
+    issue_peak = 256 * 4 * 2.4e9 / 4  # wave instructions per second
+    issue = mix["SQ_INSTS_VALU"] / 64 * cells / (mean_ms * 1e-3) / issue_peak
+    return {"bound": "fp64_valu", "kernel": kernel, "flop_per_cell": flop, "achieved": tf,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
+            "valu_issue_frac": issue, "source": "profiles/jet384_sweeps_instruction_mix.md"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,6 +347,7 @@ def main():
                      "traffic_source": "profiles/jet384_rk3_hbm_traffic.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)",
                      "bytes_per_cell_per_launch": per_cell, "mean_launch_ms": mean_ms,
                      "launches": cnt, "algorithmic_bytes": per_cell * cells},
+        "roofline_fp64": fp64_roofline(dom, cells, mean_ms, n, args.precision),
         "roofline_step": {"bytes_per_zone_update": BYTES_PER_ZU["rk3"], "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "kernels_ms": {k: {"launches": c, "mean_ms": m} for k, (c, m) in rep.items()},
