@@ -1,0 +1,128 @@
+"""Edge cases of the GPU path against the oracle: the smallest legal grid, a
+strongly anisotropic grid (x rows split into segments), near-vacuum states
+where the floors bind, a NaN cell (the reference's failure marker), the
+shortest tubes, and ragged / extreme con2prim batches."""
+import numpy as np
+import pytest
+
+from helpers import interior, per_var_rel_l1, progress, state3
+from test_gpu_parity3d import G, TOL, _run_both
+
+pytestmark = pytest.mark.gpu
+
+EXT = dict(ref=False, riemann="hllc", integrator="rk3", cfl=0.4)
+
+
+def test_smallest_grid(oracle):
+    progress("edge: 4^3 interior")
+    P = state3(4, 4, 4, seed=21)
+    Pg, Po, tg, to, nb, nbo = _run_both(oracle, P, 3, **EXT)
+    assert nb == nbo == 0
+    assert max(per_var_rel_l1(interior(Pg), interior(Po))) <= TOL
+    assert abs(tg - to) <= 1e-14 * to
+
+
+def test_anisotropic_grid_splits_rows(oracle):
+    # 5 x 6 rows of 200 cells: too few rows for the chip, so each row is
+    # streamed as several segments
+    progress("edge: 200 x 5 x 6")
+    P = state3(200, 5, 6, seed=22)
+    Pg, Po, tg, to, nb, nbo = _run_both(oracle, P, 2, **EXT,
+                                        bc=("outflow", "outflow", "periodic", "periodic", "periodic",
+                                            "periodic"))
+    assert nb == nbo == 0
+    assert max(per_var_rel_l1(interior(Pg), interior(Po))) <= TOL
+
+
+@pytest.mark.parametrize("ref", [True, False])
+def test_near_vacuum_floors(oracle, ref):
+    progress(f"edge: near vacuum ref={ref}")
+    P = state3(24, 20, 16, seed=23)
+    x = np.arange(P.shape[1])[:, None, None]
+    low = (x > 12) & (x < 20)
+    P[0] = np.where(low, 3e-7, P[0])   # a few times the 1e-7 floor
+    P[1] = np.where(low, 2e-7, P[1])
+    kw = {} if ref else EXT
+    Pg, Po, tg, to, nb, nbo = _run_both(oracle, P, 2, **kw)
+    progress(f"  unrecovered cells gpu {nb} oracle {nbo}")
+    assert nb == nbo
+    A, B = interior(Pg), interior(Po)
+    assert np.array_equal(np.isnan(A), np.isnan(B))
+    ok = np.isfinite(B).all(axis=0)
+    assert (A[0][ok] >= 1e-7).all() and (A[1][ok] >= 1e-7).all()
+    for v in range(5):
+        a, b = A[v][ok], B[v][ok]
+        assert np.abs(a - b).sum() <= TOL * np.abs(b).sum()
+
+
+def test_nan_cell_follows_reference_marker(oracle):
+    progress("edge: NaN cell")
+    P = state3(20, 16, 12, seed=24)
+    P[1, 10, 8, 6] = np.nan
+    Pg, Po, tg, to, nb, nbo = _run_both(oracle, P, 1)
+    assert nb == nbo and nb > 0
+    assert np.array_equal(np.isnan(interior(Pg)), np.isnan(interior(Po)))
+    ok = np.isfinite(interior(Po))
+    a, b = interior(Pg)[ok], interior(Po)[ok]
+    assert np.abs(a - b).sum() <= TOL * np.abs(b).sum()
+
+
+def test_shortest_tubes(oracle):
+    from test_gpu_tubes import _oracle_tubes
+    from verona_amd.tubes import Tubes, shock_tube_batch
+    progress("edge: 8-cell tubes")
+    lengths, rho, u, ux, dt, dx = shock_tube_batch(20, lmin=8, lmax=12, seed=25)
+    with Tubes(lengths) as tb:
+        tb.upload(rho, u, ux, dt, dx)
+        tb.set_method("ppm", "hllc", 50, 1e-10)
+        tb.run(30, block=64, grid=4)
+        Gt = np.stack(tb.download())
+    O = _oracle_tubes(oracle, lengths, rho, u, ux, dt, dx, 5.0 / 3.0, "ppm", "hllc", 30)
+    finite = np.isfinite(O)
+    assert np.array_equal(np.isfinite(Gt), finite)
+    for v in range(3):
+        f = finite[v]
+        assert np.abs(Gt[v][f] - O[v][f]).sum() <= 1e-12 * np.abs(O[v][f]).sum()
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000])
+def test_c2p_ragged_batches(oracle, n):
+    from verona_amd.c2p import Con2PrimBatch, c4_states, warm_guess
+    progress(f"edge: con2prim batch of {n}")
+    U, P = c4_states(n, seed=26, gamma=G)
+    Pp = warm_guess(P)
+    with Con2PrimBatch(1024, G) as cp:
+        cp.upload(U, Pp)
+        for guess in ("generic", "warm", "analytic"):
+            st = cp.solve(guess, n_iter=100, rtol=1e-12, record=True)
+            Pd, W0, it = cp.download(record=True)
+            Po, ito, cv = oracle.c2p4_solve(U, W0, G, 100, 1e-12)
+            assert st["n"] == n and st["n_unrecovered"] == 0 and cv.all()
+            assert np.array_equal(Pd, Po) and np.array_equal(it.astype(np.int32), ito)
+
+
+def test_c2p_extreme_corners(oracle):
+    """Corners of the benchmark law: Gamma = 100 cold thin gas (W within
+    5e-5 of |S|, where the W_min clamp matters), Gamma = 1 hot dense gas, and
+    mixed."""
+    from verona_amd.c2p import Con2PrimBatch, p2u
+    progress("edge: con2prim corners")
+    rows = []
+    for rho in (1e-4, 1e2):
+        for p in (1e-6, 1e2):
+            for lor in (1.0, 1.0 + 1e-12, 100.0):
+                for d in ((1, 0, 0), (0, 0, -1), (0.6, 0.0, 0.8)):
+                    ur = np.sqrt(lor * lor - 1.0)
+                    rows.append([rho, p / (G - 1.0), ur * d[0], ur * d[1], ur * d[2]])
+    P = np.array(rows)
+    U = p2u(P, G)
+    with Con2PrimBatch(1024, G) as cp:
+        cp.upload(U)
+        for guess in ("generic", "analytic"):
+            st = cp.solve(guess, n_iter=200, rtol=1e-12, record=True)
+            Pd, W0, it = cp.download(record=True)
+            Po, ito, cv = oracle.c2p4_solve(U, W0, G, 200, 1e-12)
+            assert st["n_unrecovered"] == 0 and cv.all()
+            assert np.array_equal(Pd, Po)
+            # recovered density matches the state the cells were built from
+            assert np.allclose(Pd[:, 0], P[:, 0], rtol=1e-6)
