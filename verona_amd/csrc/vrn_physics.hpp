@@ -441,14 +441,31 @@ template <int NV>
 __host__ __device__ __forceinline__ double cell_rate(const double *P, double g, const double *inv_dx)
 {
     double lor = lorentz<NV>(P);
-    double cs = sound_speed(P[0], P[1], g);
     double r = 0.0;
+#if defined(VRN_FAST)
+    // sigma and 1/(1 + sigma) do not depend on the direction (algebra as in
+    // side_speeds); max(|v + rt|, |v - rt|) = |v| + rt for rt >= 0
+    const double a = g * (g - 1.0) * P[1];
+    const double b = P[0] + g * (2.0 - g) * P[1];
+    const double dd = lor * lor * b;
+    const double s = dd + a;
+    const double rr = fdiv(1.0, lor * dd * s);
+    const double ilor = dd * s * rr, sig = a * (lor * s * rr), inv = dd * (lor * dd * rr);
+#pragma unroll
+    for (int d = 0; d < NV - 2; d++) {
+        const double v = P[2 + d] * ilor;
+        const double rt = dsqrt(sig * (1.0 - v * v + sig));
+        r = r + ((dabs(v) + rt) * inv) * inv_dx[d];
+    }
+#else
+    double cs = sound_speed(P[0], P[1], g);
 #pragma unroll
     for (int d = 0; d < NV - 2; d++) {
         double lp, lm;
         lambdas(fdiv(P[2 + d], lor), cs, lor * lor, lp, lm);
         r = r + jmax(dabs(lp), dabs(lm)) * inv_dx[d];
     }
+#endif
     return r;
 }
 
@@ -479,7 +496,13 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
         double sv = dsqrt(1.0 - v2);
         double fun = Qt + W - c * (W * (1.0 - v2) - D * sv);
         double W3 = W * W * W;
+#if defined(VRN_FAST)
+        // S2/W3 shared by both Jacobian terms: one reciprocal of W3 sv
+        const double q3 = fdiv(S2, W3 * sv);
+        double jac = 1.0 - c * (1.0 - v2) - c * (W * (2.0 * (q3 * sv)) - D * q3);
+#else
         double jac = 1.0 - c * (1.0 - v2) - c * (W * fdiv(2.0 * S2, W3) - D * fdiv(S2, W3 * sv));
+#endif
         double dW = fdiv(fun, jac);
         double Wp = W - dW;
         W = (Wp < W_min) ? 0.5 * (W + W_min) : Wp;
