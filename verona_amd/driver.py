@@ -22,21 +22,55 @@ def _kinds(comm, fun_bound, ndim):
     return kinds, (fun_bound if callable(fun_bound) else None)
 
 
-def save_dump(path, Phost, t, grid):
+def save_dump(path, Phost, t, grid, step=0):
     """Interior primitives as in SaveHDF5Parallel (src/3dim/saving.jl:57-146):
     dataset ``data`` (Julia shape (nvar, Nx, Ny[, Nz])), ``T`` and ``grid``;
-    raises on NaN like src/3dim/saving.jl:70-72."""
+    raises on NaN like src/3dim/saving.jl:70-72.  ``step`` (the number of
+    steps taken) is added so that a run can restart from the dump."""
     sl = (slice(None),) + (slice(3, -3),) * (Phost.ndim - 1)
     data = np.ascontiguousarray(Phost[sl])
     if np.isnan(data).any():
         raise RuntimeError("Nan in matrix")
-    np.savez(path, data=data, T=np.float64(t), grid=np.asarray(grid, dtype=np.float64))
+    np.savez(path, data=data, T=np.float64(t), grid=np.asarray(grid, dtype=np.float64),
+             step=np.int64(step))
+
+
+def load_dump(path):
+    """(data, T, grid, step, index) of a dump written by save_dump; the index
+    is the i of ``dump<i>.npz``."""
+    with np.load(path, allow_pickle=False) as z:
+        data, T, grid = z["data"], float(z["T"]), tuple(float(g) for g in z["grid"])
+        step = int(z["step"]) if "step" in z.files else None
+    base = os.path.basename(path)
+    if not (base.startswith("dump") and base.endswith(".npz")):
+        raise ValueError(f"not a dump file name: {base}")
+    return data, T, grid, step, int(base[4:-4])
 
 
 def harm_hll(dev, ndim, comm, P_host_view, dt, grid, Tmax, eos, drops, floor, out_dir, verbose,
              n_iter, tol, fun_bound, *, rec="wenoz", riemann="hll", integrator="rk2", cfl=0.0,
-             precision="fp64", ref_mode=True, max_steps=None):
+             precision="fp64", ref_mode=True, max_steps=None, restart=None):
+    """restart: path of a dump<i>.npz written by an earlier run of the same
+    problem; the run continues from its state, time, dump index and step
+    count (the reference has no restart; this is the natural one for its
+    dump format)."""
     kinds, callback = _kinds(comm if comm is not None else Cart(), fun_bound, ndim)
+    t = 0.0
+    i = 0
+    steps = 0
+    if restart is not None:
+        data, t, g0, step0, i = load_dump(restart)
+        if step0 is None:
+            raise ValueError("dump has no step count; it was not written by this driver")
+        if any(abs(a - b) > 0 for a, b in zip(g0, grid)):
+            raise ValueError(f"dump grid {g0} differs from the run's {tuple(grid)}")
+        Ph = dev.download("P")
+        sl = (slice(None),) + (slice(3, -3),) * ndim
+        if Ph[sl].shape != data.shape:
+            raise ValueError(f"dump shape {data.shape} differs from the grid's {Ph[sl].shape}")
+        Ph[sl] = data
+        dev.upload("P", Ph)
+        steps = step0
     nozzle = getattr(fun_bound, "nozzle", None)
     phi_seq = None
     if nozzle is not None:
@@ -54,16 +88,18 @@ def harm_hll(dev, ndim, comm, P_host_view, dt, grid, Tmax, eos, drops, floor, ou
                  precision=precision, n_iter=n_iter, tol=tol, floor=floor, dt=dt, cfl=cfl,
                  dx=dx, dy=dy, dz=dz,
                  bc=[("none" if (callback is not None and k != "periodic") else k) for k in kinds])
-    t = 0.0
-    i = 0
     coords = (0,) * ndim
     dims = (1,) * ndim
     if out_dir is not None:
         os.makedirs(out_dir, exist_ok=True)
-        save_dump(os.path.join(out_dir, f"dump{i}.npz"), dev.download("P"), t, grid)
+        if restart is None:
+            save_dump(os.path.join(out_dir, f"dump{i}.npz"), dev.download("P"), t, grid)
+    # the threshold sequence drops, 2 drops, ... accumulated in FP64 as the
+    # straight run accumulates it
     thres = drops
+    for _ in range(i):
+        thres += drops
     t0 = time.time()
-    steps = 0
     ncell = int(np.prod([s for s in dev.shape[1:]]))
     while t < Tmax:
         if max_steps is not None and steps >= max_steps:
@@ -89,6 +125,6 @@ def harm_hll(dev, ndim, comm, P_host_view, dt, grid, Tmax, eos, drops, floor, ou
                 print(f"speed: {ncell * drops / dt / el * 1e-6:.3g} zones [10^6/s]")
                 t0 = time.time()
             if out_dir is not None:
-                save_dump(os.path.join(out_dir, f"dump{i}.npz"), dev.download("P"), t, grid)
+                save_dump(os.path.join(out_dir, f"dump{i}.npz"), dev.download("P"), t, grid, steps)
     dev.sync()
     return i
