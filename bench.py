@@ -121,6 +121,32 @@ def run_kh(n=2048, steps=40, warmup=3, precision="fp64fast"):
             "ms_per_step": el / steps * 1e3, "n_unrecovered": nbad}
 
 
+def run_tubes(ntubes=4096, steps=500):
+    """Batched uneven 1D shock tubes: 4096 tubes, lengths U{256..8192} (SplitMix64 seed
+    42), PPM + HLLC, RK2, dt = 0.4 dx per tube, one persistent hipGraph launch with the
+    autotuned launch shape.  value = cells x steps / s of the timed launch."""
+    from verona_amd.tubes import Tubes, shock_tube_batch
+    lengths, rho, u, ux, dt, dx = shock_tube_batch(ntubes, 256, 8192, seed=42)
+    with Tubes(lengths) as tb:
+        tb.upload(rho, u, ux, dt, dx)
+        tb.set_method("ppm", "hllc", 50, 1e-10)
+        block, grid = tb.autotune(probe_steps=2)
+        tb.upload(rho, u, ux, dt, dx)
+        tb.run(2, block, grid)  # graph capture + warm-up
+        tb.upload(rho, u, ux, dt, dx)
+        ms = tb.run(steps, block, grid)
+        r, _, _ = tb.download()
+    cells = int(lengths.sum())
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    # Verona1D's con2prim has no floors: tubes whose random states it cannot
+    # recover end with NaN markers (the oracle marks the same cells)
+    bad = int(sum(not np.isfinite(r[off[t]:off[t + 1]]).all() for t in range(ntubes)))
+    return {"what": f"{ntubes} uneven 1D tubes ({cells} cells), PPM+HLLC, RK2, {steps} steps, "
+                    "one persistent hipGraph launch", "value": cells * steps / (ms * 1e-3),
+            "unit": "cell-steps/s", "ms": ms, "launch_shape": [block, grid],
+            "tubes_with_nan_markers": bad}
+
+
 def fp32_error(n, steps):
     """Relative L1 per variable of the FP32 reconstruction/Riemann form against the
     all-FP64 (exact) form after `steps` steps of the jet from the same state."""
@@ -371,6 +397,8 @@ def main():
         out["fp32_form"]["error_vs_fp64"] = fp32_error(n, 10)
         log("kelvin-helmholtz")
         out["kelvin_helmholtz"] = run_kh()
+        log("batched tubes")
+        out["batched_tubes"] = run_tubes()
     if not args.no_c2p:
         out["con2prim"] = run_c2p(log=log)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
