@@ -1,4 +1,4 @@
-"""Run-loop drop-in (Verona3D.HARM_HLL over the fused step) and restart from
+"""Run-loop drop-in (Verona3D/2D.HARM_HLL over the fused step) and restart from
 a dump: a run continued from dump<i>.npz reaches the same final state as the
 uninterrupted run.  The dump holds P only (as the reference's does), so the
 restarted run rebuilds U = PtoU(P); con2prim's round trip is not bit-exact,
@@ -51,3 +51,44 @@ def test_restart_from_dump_continues_the_run(tmp_path):
     assert max(errs) <= 1e-12
     # dumps before the restart point were not rewritten
     assert not os.path.exists(os.path.join(b, "dump0.npz"))
+
+
+def test_verona2d_run_loop_matches_oracle(oracle, tmp_path):
+    """Verona2D.HARM_HLL (src/2dim/algos.jl:192-324) as a drop-in: periodic
+    Kelvin-Helmholtz at 64^2 in the reference's own method (WENOZ + HLL,
+    midpoint RK2, fixed dt); the last dump equals the oracle after the same
+    number of steps, and the dump cadence follows `t > thres`."""
+    from helpers import from_oracle2, to_oracle2
+    from verona_amd import Cart, EosTypes, Verona2D
+    from verona_amd.problems import kelvin_helmholtz
+    progress("Verona2D run loop")
+    n = 64
+    dx = 1.0 / n
+    dt = 0.4 / (1 / dx + 1 / dx)
+    Tmax, drops = 40 * dt, 10 * dt
+    P = Verona2D.ParVector2D(n, n)
+    P.arr[...] = kelvin_helmholtz(n, gamma=G)
+    P0 = P.arr.copy()
+    out = str(tmp_path / "kh")
+    ndump = Verona2D.HARM_HLL(Cart((True, True)), P, 1, 1, 4, 4, dt, dx, dx, Tmax, EosTypes.Polytrope(G), drops,
+                              1e-7, out, 50, 1e-10)
+    # the reference's loop: t += dt while t < Tmax, dump when t > thres
+    t, steps, thres, i = 0.0, 0, drops, 0
+    while t < Tmax:
+        t += dt
+        steps += 1
+        if t > thres:
+            i += 1
+            thres += drops
+    assert ndump == i
+    Po = to_oracle2(P0)
+    m = oracle.make_mode(ref=True, gamma=G, floor=1e-7, dt=dt, dx=dx, dy=dx, n_iter=50, tol=1e-10)
+    bad, _, _ = oracle.run2(Po, m, [1, 1, 1, 1], steps)
+    assert bad == 0
+    with np.load(os.path.join(out, f"dump{ndump}.npz")) as z:
+        data, T = z["data"], float(z["T"])
+    ref = from_oracle2(Po)[:, 3:-3, 3:-3]
+    assert abs(T - t) == 0.0
+    errs = per_var_rel_l1(data, ref)
+    progress(f"  {steps} steps, {ndump} dumps, per-var rel L1 {['%.1e' % e for e in errs]}")
+    assert max(errs) <= 1e-12
