@@ -321,8 +321,9 @@ def main():
                     help="FP64 form of the headline run (both agree with the oracle to <= 1e-12)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the grid would shard).", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (the grid would shard by "
+              "Cartesian domain decomposition with 3-cell halos, one exchange per RK stage).",
+              file=sys.stderr)
         sys.exit(2)
     n, K, W = args.n, args.steps, args.warmup
     if args.copy_calibration:
