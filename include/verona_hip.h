@@ -171,6 +171,15 @@ int vrn_timing_enable(vrn_ctx *ctx, int enable);
 int vrn_timing_report(vrn_ctx *ctx, char *names, int name_len, int64_t *counts, double *mean_ms,
                       int cap, int *n_out);
 
+/* Work partition of the fused sweeps (no counterpart in the reference; it
+ * changes no arithmetic, every face and cell is computed once either way):
+ * x sweep: rows_per_wave whole rows per wavefront, or, with xseg > 1, one of
+ * xseg segments of a row per wavefront (then rows_per_wave must be 1);
+ * y / z sweeps: seg_len interior cells per marching segment.  vrn_create
+ * chooses a partition for the grid; vrn_get_partition reads it back. */
+int vrn_set_partition(vrn_ctx *ctx, int rows_per_wave, int xseg, int seg_len);
+int vrn_get_partition(vrn_ctx *ctx, int *rows_per_wave, int *xseg, int *seg_len);
+
 
 /* ------------------------------------------------------------------------
  * Batched 1D shock tubes: many independent Verona1D.HARM_HLL problems

@@ -1035,6 +1035,33 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     return 0;
 }
 
+int vrn_set_partition(vrn_ctx *c, int rows_per_wave, int xseg, int seg_len)
+{
+    if (int r = check_ctx(c)) return r;
+    const int Nxi = c->G.NX - 6;
+    if (rows_per_wave < 1 || rows_per_wave > 64 || xseg < 1 || (xseg > 1 && rows_per_wave != 1) ||
+        xseg > Nxi || seg_len < 8 || seg_len > 4096)
+        return fail(VRN_E_ARG, "bad sweep partition (rows_per_wave %d, xseg %d, seg_len %d)", rows_per_wave,
+                    xseg, seg_len);
+    // a row segment streams at least 34 cells (x0-1 .. x1 with >= 32 interior
+    // cells) unless the row itself is shorter
+    if (xseg > 1 && (Nxi + xseg - 1) / xseg < 32)
+        return fail(VRN_E_ARG, "row segments shorter than 32 cells (nx %d, xseg %d)", Nxi, xseg);
+    c->rows_per_wave = rows_per_wave;
+    c->xseg = xseg;
+    c->seg_len = seg_len;
+    return 0;
+}
+
+int vrn_get_partition(vrn_ctx *c, int *rows_per_wave, int *xseg, int *seg_len)
+{
+    if (int r = check_ctx(c)) return r;
+    if (rows_per_wave) *rows_per_wave = c->rows_per_wave;
+    if (xseg) *xseg = c->xseg;
+    if (seg_len) *seg_len = c->seg_len;
+    return 0;
+}
+
 int vrn_timing_enable(vrn_ctx *c, int enable)
 {
     if (int r = check_ctx(c)) return r;

@@ -138,6 +138,16 @@ class DeviceGrid:
               "vrn_step")
         return dt.value if want_dt else None
 
+    # ------------------------------------------------------------------ work partition
+    def set_partition(self, rows_per_wave, xseg, seg_len):
+        check(self.lib.vrn_set_partition(self.h, int(rows_per_wave), int(xseg), int(seg_len)),
+              "vrn_set_partition")
+
+    def partition(self):
+        r, x, s = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.vrn_get_partition(self.h, C.byref(r), C.byref(x), C.byref(s)), "vrn_get_partition")
+        return r.value, x.value, s.value
+
     # ------------------------------------------------------------------ timing
     def timing(self, enable=True):
         check(self.lib.vrn_timing_enable(self.h, int(enable)), "vrn_timing_enable")
