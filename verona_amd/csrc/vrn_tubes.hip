@@ -146,12 +146,20 @@ __device__ void tube_stage(const TubeArgs &a, int tb, int s, double *lds)
     const int T = blockDim.x, tid = threadIdx.x;
     int rlo, rhi;
     rec_range<REC>(N, rlo, rhi);
-    // LDS: PL[T+1][3] (slot 0 = carry), F[T+1][3] (slot 0 = carry)
+    // LDS: PL[2][T][3] and F[2][T][3], alternating halves per chunk.  A lane
+    // reads its left neighbour's PL and F at position tid-1 of its own half,
+    // lane 0 the previous chunk's last lane in the other half.  Races: the
+    // next chunk writes the other half's PL only after this chunk's second
+    // barrier (all PL reads done) and its F only after its own first barrier,
+    // which no wave reaches before finishing this chunk's F reads.
     double *sPL = lds;
-    double *sF = lds + 3 * (T + 1);
-    for (int c0 = 0; c0 <= N; c0 += T) {
+    double *sF = lds + 6 * T;
+    int h = 0;
+    for (int c0 = 0; c0 <= N; c0 += T, h ^= 1) {
         const int c = c0 + tid;  // stream element: recon of cell c, face c-1, cell c-1
         const bool in = c <= N;
+        const int me = h * T + tid;
+        const int prev = tid > 0 ? me - 1 : (h ^ 1) * T + T - 1;
         double PLc[3], PRm[3];  // PL[c], PR[c-1]
         if (in && c >= rlo && c < rhi) {
 #pragma unroll
@@ -173,18 +181,18 @@ __device__ void tube_stage(const TubeArgs &a, int tb, int s, double *lds)
             }
         }
 #pragma unroll
-        for (int v = 0; v < 3; v++) sPL[3 * (tid + 1) + v] = PLc[v];
+        for (int v = 0; v < 3; v++) sPL[3 * me + v] = PLc[v];
         __syncthreads();
         double F[3] = {0.0, 0.0, 0.0};
         const bool face = in && c >= 1;  // face c-1 (between cells c-1 and c)
         if (face) {
             double PLm[3];
 #pragma unroll
-            for (int v = 0; v < 3; v++) PLm[v] = sPL[3 * tid + v];
+            for (int v = 0; v < 3; v++) PLm[v] = sPL[3 * prev + v];
             face1<RIEM>(PLm, PRm, g, F);
         }
 #pragma unroll
-        for (int v = 0; v < 3; v++) sF[3 * (tid + 1) + v] = F[v];
+        for (int v = 0; v < 3; v++) sF[3 * me + v] = F[v];
         __syncthreads();
         if (face) {
             const int j = c - 1;  // cell updated / recovered by this lane
@@ -193,7 +201,7 @@ __device__ void tube_stage(const TubeArgs &a, int tb, int s, double *lds)
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 double ub = a.U[v][o + j];
-                Uo[v] = upd ? ub - coef * (F[v] - sF[3 * tid + v]) : ub;
+                Uo[v] = upd ? ub - coef * (F[v] - sF[3 * prev + v]) : ub;
                 if (s == 0)
                     a.Uh[v][o + j] = Uo[v];
                 else if (upd)
@@ -209,16 +217,6 @@ __device__ void tube_stage(const TubeArgs &a, int tb, int s, double *lds)
                     a.P[v][o + j] = x[v];
             }
         }
-        __syncthreads();
-        // carry the last lane's PL and F into slot 0 for the next chunk
-        if (tid == T - 1) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                sPL[v] = sPL[3 * T + v];
-                sF[v] = sF[3 * T + v];
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -430,7 +428,7 @@ int vrn_tubes_set_method(vrn_tubes *c, int rec, int riemann, int n_iter, double 
 
 static void launch_tubes(vrn_tubes *c, int block, int grid)
 {
-    size_t lds = sizeof(double) * 6 * (block + 1);
+    size_t lds = sizeof(double) * 12 * block;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, c->s, c->queue);
 #define VRN_TL(R_, M_) hipLaunchKernelGGL((k_tubes<R_, M_>), dim3(grid), dim3(block), lds, c->s, c->args)
     if (c->riemann == VRN_RIEMANN_HLL) {
