@@ -78,6 +78,9 @@ __device__ __forceinline__ void face1(const double *PL, const double *PR, double
 __device__ __forceinline__ void c2p1(const double *U, double *x, double g, int n_iter, double tol)
 {
     for (int it = 0; it < n_iter; it++) {
+        // once all three unknowns are NaN every further iteration returns NaN:
+        // stopping here gives the same result as running out the iterations
+        if (__builtin_isnan(x[0]) && __builtin_isnan(x[1]) && __builtin_isnan(x[2])) break;
         double gam = dsqrt(x[2] * x[2] + 1.0);
         double w = g * x[1] + x[0];
         double f0 = gam * x[0] - U[0];
