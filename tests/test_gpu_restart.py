@@ -92,3 +92,39 @@ def test_verona2d_run_loop_matches_oracle(oracle, tmp_path):
     errs = per_var_rel_l1(data, ref)
     progress(f"  {steps} steps, {ndump} dumps, per-var rel L1 {['%.1e' % e for e in errs]}")
     assert max(errs) <= 1e-12
+
+
+def test_host_callback_boundary_equals_declarative(tmp_path):
+    """fun_bound as a plain callable (run on a host copy of P at the start of
+    every step, as src/3dim/algos.jl:276 calls it) with the zero-gradient
+    copies of examples/Jet3D/HARM_jet_breakout.jl:136-157 gives the same
+    result as the declarative Boundary(outflow=...) the library applies on
+    the device."""
+    from verona_amd import Boundary, Cart, EosTypes, Verona3D
+    progress("fun_bound callback vs declarative outflow")
+    P = state3(20, 16, 12, seed=33)
+    nx, ny, nz = 20, 16, 12
+    dt = 0.4 / (nx + ny + nz)
+    faces = ("xlo", "xhi", "ylo", "yhi", "zlo", "zhi")
+
+    def outflow(Ph, t, coords, dims):
+        # whole-plane copies, x then y then z
+        Ph[:, 0:3] = Ph[:, 3:4]
+        Ph[:, -3:] = Ph[:, -4:-3]
+        Ph[:, :, 0:3] = Ph[:, :, 3:4]
+        Ph[:, :, -3:] = Ph[:, :, -4:-3]
+        Ph[:, :, :, 0:3] = Ph[:, :, :, 3:4]
+        Ph[:, :, :, -3:] = Ph[:, :, :, -4:-3]
+
+    finals = []
+    for fb in (outflow, Boundary(outflow=faces)):
+        Pv = Verona3D.ParVector3D(nx, ny, nz)
+        Pv.arr[...] = P
+        CuP = Verona3D.CuParVector3D(Pv, gamma=G)
+        try:
+            Verona3D.HARM_HLL(Cart((False, False, False)), CuP, (1, 1, 1), (4, 4, 4), dt, 1 / nx, 1 / ny, 1 / nz,
+                              6 * dt, EosTypes.Polytrope(G), 10.0, 1e-7, None, False, 50, 1e-10, fb)
+            finals.append(CuP.arr[:, 3:-3, 3:-3, 3:-3])
+        finally:
+            CuP.dev.close()
+    assert np.array_equal(finals[0], finals[1])
