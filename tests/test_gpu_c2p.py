@@ -33,6 +33,19 @@ def _rel_l1(a, b):
     return np.abs(a - b).sum(0) / np.abs(b).sum(0)
 
 
+def check_counters(st, it):
+    """The solve's statistics (iteration histogram, maximum, per-wave maxima
+    over 64-cell chunks) equal what its per-cell iteration counts give."""
+    it = it.astype(np.int64)
+    assert st["iters_max"] == it.max()
+    assert abs(st["iters_mean"] - it.mean()) < 1e-9
+    h = np.bincount(np.minimum(it, 63), minlength=64)
+    h[0] = 0
+    assert np.array_equal(np.array(st["hist"]), h)
+    wm = np.concatenate([it, np.zeros((-len(it)) % 64, np.int64)]).reshape(-1, 64).max(1)
+    assert abs(st["wave_max_mean"] - wm.mean()) < 1e-12
+
+
 def test_features_match_oracle(cp, oracle):
     progress("c2p features")
     U, P = c4_states(1 << 16, seed=3, gamma=G)
@@ -113,7 +126,7 @@ def test_plain_guess_bit_exact_vs_oracle(cp, oracle, guess):
     assert st["n_unrecovered"] == 0 and st["n_fallback"] == int((~cv).sum()) == 0
     assert np.array_equal(Pd, Po)
     assert np.array_equal(it.astype(np.int32), ito)
-    assert abs(st["iters_mean"] - ito.mean()) < 1e-9
+    check_counters(st, it)
 
 
 def test_learned_guess_training_and_gain(cp, oracle):
@@ -145,6 +158,7 @@ def test_learned_guess_training_and_gain(cp, oracle):
     # and with the oracle from the same initial W, bit for bit
     Po, ito, cv = oracle.c2p4_solve(U, W0, G, 100, RTOL)
     assert cv.all() and np.array_equal(Pl, Po) and np.array_equal(it.astype(np.int32), ito)
+    check_counters(sl, it)
     # the gain: fewer Newton iterations than either plain start
     assert sl["iters_mean"] < sa["iters_mean"]
     assert sl["wave_max_mean"] < 0.5 * sg["wave_max_mean"]

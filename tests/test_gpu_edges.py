@@ -87,6 +87,7 @@ def test_shortest_tubes(oracle):
 
 @pytest.mark.parametrize("n", [1, 63, 1000])
 def test_c2p_ragged_batches(oracle, n):
+    from test_gpu_c2p import check_counters
     from verona_amd.c2p import Con2PrimBatch, c4_states, warm_guess
     progress(f"edge: con2prim batch of {n}")
     U, P = c4_states(n, seed=26, gamma=G)
@@ -99,6 +100,7 @@ def test_c2p_ragged_batches(oracle, n):
             Po, ito, cv = oracle.c2p4_solve(U, W0, G, 100, 1e-12)
             assert st["n"] == n and st["n_unrecovered"] == 0 and cv.all()
             assert np.array_equal(Pd, Po) and np.array_equal(it.astype(np.int32), ito)
+            check_counters(st, it)
 
 
 def test_c2p_extreme_corners(oracle):

@@ -85,20 +85,27 @@ __device__ __forceinline__ Feat c4_features(double D, double Qt, double S2)
     Feat f;
     f.E = -Qt;
     const double tau = f.E - D;
-    const double rE = 1.0 / f.E;
+    // 1/E: hardware reciprocal + two Newton-Raphson steps (about 1 ulp; E is
+    // a normal positive number wherever the features are used)
+    double rE = __builtin_amdgcn_rcp(f.E);
+    rE = __builtin_fma(__builtin_fma(-f.E, rE, 1.0), rE, rE);
+    rE = __builtin_fma(__builtin_fma(-f.E, rE, 1.0), rE, rE);
     // E^2 - S^2 - D^2 = tau (E + D) - S^2, in FP64: it is a difference of
     // nearly equal terms when p << E
     f.z = (tau * (f.E + D) - S2) * (rE * rE);
     f.ok = D > 0.0 && tau > 0.0 && f.z > 0.0 && f.z < 1.0 && __builtin_isfinite(f.E) &&
            __builtin_isfinite(S2);
     if (f.ok) {
+        // single hardware instructions (v_log_f32 = log2, v_sqrt_f32,
+        // v_rcp_f32; ~1 ulp): the features only steer the initial guess
+        const float LN2 = 0.693147181f;
         const float tf = (float)tau;
-        const float Sm = __builtin_sqrtf((float)S2);
-        f.x[0] = 0.1f * __logf((float)D);
-        f.x[1] = 0.1f * __logf(tf);
-        f.x[2] = 0.2f * __logf(1.0f + __fdividef(Sm, tf));
+        const float Sm = __builtin_amdgcn_sqrtf((float)S2);
+        f.x[0] = (0.1f * LN2) * __builtin_amdgcn_logf((float)D);
+        f.x[1] = (0.1f * LN2) * __builtin_amdgcn_logf(tf);
+        f.x[2] = (0.2f * LN2) * __builtin_amdgcn_logf(1.0f + Sm * __builtin_amdgcn_rcpf(tf));
         f.x[3] = Sm * (float)rE;
-        f.x[4] = __builtin_sqrtf((float)f.z);
+        f.x[4] = __builtin_amdgcn_sqrtf((float)f.z);
     } else {
 #pragma unroll
         for (int k = 0; k < NIN; k++) f.x[k] = 0.0f;
@@ -122,12 +129,13 @@ __device__ __forceinline__ bool c4_targets(const double *P, double g, const Feat
 // Initial W from the network's first output.
 __device__ __forceinline__ double learned_w0(const Feat &f, float y0)
 {
-    float qh = (float)f.z * __expf(y0);
+    const float LOG2E = 1.44269504f;
+    float qh = (float)f.z * __builtin_amdgcn_exp2f(y0 * LOG2E);
     qh = qh < 0.0f ? 0.0f : (qh > 0.5f ? 0.5f : qh);
     // expm1(qh): Taylor to 4th order below 1/16 (relative error < 2e-7),
     // exp - 1 above
     const float em = qh < 0.0625f ? qh * (1.0f + qh * (0.5f + qh * (1.0f / 6.0f + qh * (1.0f / 24.0f))))
-                                  : __expf(qh) - 1.0f;
+                                  : __builtin_amdgcn_exp2f(qh * LOG2E) - 1.0f;
     return f.E + f.E * (double)em;
 }
 
@@ -192,6 +200,54 @@ __device__ __forceinline__ long long wave_sum_ll(long long v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
+// Wave maximum of non-negative counts, built bit by bit from the top with one
+// ballot per bit (counts below 16 take five compares).
+__device__ __forceinline__ int wave_max_count(int v)
+{
+    if (__ballot(v >= 16)) return wave_max_i(v);
+    int m = 0;
+#pragma unroll
+    for (int b = 3; b >= 0; b--) {
+        const int c = m | (1 << b);
+        if (__ballot(v >= c)) m = c;
+    }
+    return m;
+}
+
+// Per-lane histogram of iteration counts 1..15: 8-bit fields, bins 0..7 in
+// w0 and 8..15 in w1.  flush() adds the wave's totals to the block bins; it
+// must run (wave-uniformly) before any lane has counted 256 cells.
+struct LaneHist {
+    unsigned long long w0 = 0, w1 = 0;
+    __device__ __forceinline__ void add(int it)
+    {
+        const unsigned long long inc = 1ull << (8 * (it & 7));
+        if (it < 8)
+            w0 += inc;
+        else
+            w1 += inc;
+    }
+    __device__ __forceinline__ void flush(unsigned *bins, int lane)
+    {
+        // widen to 16-bit fields (even and odd bytes) so that the wave sums
+        // (at most 64 x 255) cannot carry into the next field
+        const unsigned long long M = 0x00ff00ff00ff00ffull;
+        const unsigned long long t[4] = {
+            (unsigned long long)wave_sum_ll((long long)(w0 & M)), (unsigned long long)wave_sum_ll((long long)((w0 >> 8) & M)),
+            (unsigned long long)wave_sum_ll((long long)(w1 & M)), (unsigned long long)wave_sum_ll((long long)((w1 >> 8) & M))};
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int bin = 8 * (k >> 1) + 2 * j + (k & 1);
+                    const unsigned c = (unsigned)((t[k] >> (16 * j)) & 0xffffu);
+                    if (bin >= 1 && c) atomicAdd(&bins[bin], c);
+                }
+        }
+        w0 = w1 = 0;
+    }
+};
 
 struct Counters {
     unsigned long long n_unrecovered, n_fallback, n_guess_bad, iters_sum, wave_max_sum, n_waves;
@@ -210,6 +266,8 @@ struct Rescue {
     double W0;  // its initial W
 };
 
+struct NetRegs;
+
 struct SolveArgs {
     long n;
     const double *U;      // SoA: plane v at U + v * n
@@ -223,7 +281,7 @@ struct SolveArgs {
     double g, rtol;
     int n_iter;
     int record;
-    const float *params;  // FP32 parameters (learned guess)
+    const NetRegs *net;   // per-lane weight fragments (learned guess), k_pack_net
 };
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
@@ -234,7 +292,6 @@ typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 // v_cvt_pk_bf16_f32 on gfx950, with the hazard wait states the compiler
 // inserts after a matrix-core result).
 typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned cvt_pk_bf16(float lo, float hi)
 {
@@ -251,9 +308,10 @@ __device__ __forceinline__ unsigned relu_pk_bf16(unsigned x)
 // ReLU + BF16 of an accumulator tile's four values, as one 64-bit word.
 __device__ __forceinline__ unsigned long long relu_pack4(const f32x4 &v)
 {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, __builtin_convertvector(v, bf16x4v));
-    const unsigned lo = relu_pk_bf16((unsigned)b);
-    const unsigned hi = relu_pk_bf16((unsigned)(b >> 32));
+    // pairwise: a 4-wide vector conversion is lowered to one conversion per
+    // value plus permutes
+    const unsigned lo = relu_pk_bf16(cvt_pk_bf16(v[0], v[1]));
+    const unsigned hi = relu_pk_bf16(cvt_pk_bf16(v[2], v[3]));
     return ((unsigned long long)hi << 32) | lo;
 }
 __device__ __forceinline__ bf16x8 cat8(unsigned long long lo, unsigned long long hi)
@@ -338,6 +396,15 @@ __device__ __forceinline__ void load_net(const float *prm, NetRegs &nr, int lane
     for (int j = 0; j < 4; j++) nr.b3[j] = (g == 0 && j < NOUT) ? prm[OFF_B3 + j] : 0.0f;
 }
 
+// The fragments are formed once per parameter set (one wave) and read by
+// every wave of the inference kernels.
+__global__ void __launch_bounds__(64) k_pack_net(const float *__restrict__ prm, NetRegs *__restrict__ net)
+{
+    NetRegs nr;
+    load_net(prm, nr, threadIdx.x);
+    net[threadIdx.x] = nr;
+}
+
 // Network outputs for the 64 cells of a wave; lane c passes the features of
 // cell c and receives that cell's outputs (y1 only when BOTH).  All 64 lanes
 // must be active.
@@ -391,9 +458,9 @@ __global__ void __launch_bounds__(256, 3) k_solve(SolveArgs a)
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < 64; i += blockDim.x) hist[i] = 0;
-    unsigned hcnt = 0;  // lane b < 16: cells of this wave that took b iterations
+    LaneHist lh;
     NetRegs nr;
-    if constexpr (GUESS == VRN_GUESS_LEARNED) load_net(a.params, nr, lane);
+    if constexpr (GUESS == VRN_GUESS_LEARNED) nr = a.net[lane];
     __syncthreads();
     const long n = a.n;
     const long nchunks = (n + 63) / 64;
@@ -480,24 +547,22 @@ __global__ void __launch_bounds__(256, 3) k_solve(SolveArgs a)
             }
             isum += it;
             imax = it > imax ? it : imax;
-            if (it >= 16) atomicAdd(&hist[it < 63 ? it : 63], 1u);
+            if (it >= 16)
+                atomicAdd(&hist[it < 63 ? it : 63], 1u);
+            else if (it >= 1)
+                lh.add(it);
         }
-        // histogram of the common counts by ballot: lane b keeps bin b
         const int mine = act ? it : 0;
-#pragma unroll
-        for (int b = 1; b < 16; b++) {
-            const unsigned c = (unsigned)__popcll(__ballot(mine == b));
-            hcnt += lane == b ? c : 0u;
-        }
-        const int wm = wave_max_i(mine);
+        const int wm = wave_max_count(mine);
         wmax_sum += wm;
         nw++;
+        if ((nw & 127) == 0) lh.flush(hist, lane);  // nw is wave-uniform; fields stay below 256
     }
+    lh.flush(hist, lane);
     // per-wave totals -> block row (no global atomics on shared addresses)
     const long long t_unrec = wave_sum_ll((long long)unrec), t_fall = wave_sum_ll((long long)fall),
                     t_bad = wave_sum_ll((long long)bad), t_isum = wave_sum_ll((long long)isum);
     const int t_imax = wave_max_i(imax);
-    if (lane >= 1 && lane < 16 && hcnt) atomicAdd(&hist[lane], hcnt);
     if (lane == 0) {
         wtot[wid][0] = (unsigned long long)t_unrec;
         wtot[wid][1] = (unsigned long long)t_fall;
@@ -639,12 +704,11 @@ __global__ void k_features(long n, const double *__restrict__ U, const double *_
 }
 
 // Inference of the network alone on given features (64 samples per wave).
-__global__ void __launch_bounds__(256) k_infer(long n, const float *__restrict__ X, const float *prm,
+__global__ void __launch_bounds__(256) k_infer(long n, const float *__restrict__ X, const NetRegs *net,
                                                float *__restrict__ Y)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    NetRegs nr;
-    load_net(prm, nr, lane);
+    const NetRegs nr = net[lane];
     const long nchunks = (n + 63) / 64;
     for (long ch = (long)blockIdx.x * 4 + wid; ch < nchunks; ch += (long)gridDim.x * 4) {
         const long i = ch * 64 + lane;
@@ -854,6 +918,8 @@ struct vrn_c2p {
     Rescue *rescue = nullptr;
     unsigned long long *crow = nullptr;  // per-block counter rows
     float *params = nullptr;  // device FP32 parameters
+    NetRegs *net = nullptr;   // their per-lane MFMA fragments (64 lanes)
+    bool net_ok = false;      // net matches params
     bool have_params = false, have_prev = false, have_P = false, recorded = false;
     // training / test buffers (allocated on first use)
     float *cache = nullptr, *part = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr, *tprm = nullptr;
@@ -865,6 +931,17 @@ struct vrn_c2p {
 };
 
 namespace {
+
+// Weight fragments of the current parameters, formed on the stream before
+// the first inference that needs them.
+int ensure_net(vrn_c2p *h)
+{
+    if (h->net_ok) return 0;
+    hipLaunchKernelGGL(k_pack_net, dim3(1), dim3(64), 0, h->s, (const float *)h->params, h->net);
+    HIPCHK(hipGetLastError());
+    h->net_ok = true;
+    return 0;
+}
 
 int ensure_train(vrn_c2p *h)
 {
@@ -978,7 +1055,8 @@ int vrn_c2p_create(int64_t n_max, double gamma, vrn_c2p **out)
         hipMalloc(&h->iters, n_max) != hipSuccess || hipMalloc(&h->cnt, sizeof(Counters)) != hipSuccess ||
         hipMalloc(&h->rescue, sizeof(Rescue) * n_max) != hipSuccess ||
         hipMalloc(&h->crow, sizeof(unsigned long long) * PART_STRIDE * MAX_SOLVE_GRID) != hipSuccess ||
-        hipMalloc(&h->params, sizeof(float) * NPARAM) != hipSuccess)
+        hipMalloc(&h->params, sizeof(float) * NPARAM) != hipSuccess ||
+        hipMalloc(&h->net, sizeof(NetRegs) * 64) != hipSuccess)
         return cleanup(fail(-2, "vrn_c2p_create: device allocation of %zu bytes failed", 4 * nb));
     *out = h;
     return 0;
@@ -998,6 +1076,7 @@ int vrn_c2p_destroy(vrn_c2p *h)
     (void)hipFree(h->rescue);
     (void)hipFree(h->crow);
     (void)hipFree(h->params);
+    (void)hipFree(h->net);
     (void)hipFree(h->cache);
     (void)hipFree(h->part);
     (void)hipFree(h->grad);
@@ -1053,6 +1132,8 @@ int vrn_c2p_solve(vrn_c2p *h, int guess, int n_iter, double rtol, vrn_c2p_stats 
     if (guess == VRN_GUESS_LEARNED && !h->have_params) return fail(-1, "vrn_c2p_solve: learned guess needs trained parameters (vrn_mlp_set / vrn_mlp_train)");
     if (n_iter < 1 || n_iter > 10000) return fail(-1, "vrn_c2p_solve: n_iter %d out of range", n_iter);
     if (!(rtol > 0.0 && rtol < 1.0)) return fail(-1, "vrn_c2p_solve: rtol %g out of range", rtol);
+    if (guess == VRN_GUESS_LEARNED)
+        if (int rc = ensure_net(h)) return rc;
     SolveArgs a;
     a.n = h->n;
     a.U = h->U;
@@ -1067,7 +1148,7 @@ int vrn_c2p_solve(vrn_c2p *h, int guess, int n_iter, double rtol, vrn_c2p_stats 
     a.rtol = rtol;
     a.n_iter = n_iter;
     a.record = record;
-    a.params = h->params;
+    a.net = h->net;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(128), 0, h->s, h->cnt);
     const long nchunks = (h->n + 63) / 64;
     const int grid = grid_for(nchunks, 4, MAX_SOLVE_GRID);
@@ -1150,6 +1231,7 @@ int vrn_mlp_set(vrn_c2p *h, const float *params)
     HIPCHK(hipMemcpyAsync(h->params, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
     h->have_params = true;
+    h->net_ok = false;
     return 0;
 }
 
@@ -1186,10 +1268,11 @@ int vrn_mlp_infer(vrn_c2p *h, int64_t n, const float *X, float *Y)
     if (n < 1 || n > h->n_max) return fail(-1, "vrn_mlp_infer: n = %lld out of range", (long long)n);
     if (!h->have_params) return fail(-1, "vrn_mlp_infer: no parameters set or trained");
     if (int rc = ensure_pairs(h, n)) return rc;
+    if (int rc = ensure_net(h)) return rc;
     HIPCHK(hipMemcpyAsync(h->X, X, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, h->s));
     const long nchunks = (n + 63) / 64;
     hipLaunchKernelGGL(k_infer, dim3(grid_for(nchunks, 4, 4096)), dim3(256), 0, h->s, (long)n, (const float *)h->X,
-                       (const float *)h->params, h->T);
+                       (const NetRegs *)h->net, h->T);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(Y, h->T, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
@@ -1246,6 +1329,7 @@ int vrn_mlp_train(vrn_c2p *h, int64_t n, const double *U, const double *P, const
     } else {
         he_init(w0, tp->seed);
     }
+    h->net_ok = false;
     HIPCHK(hipMemcpyAsync(h->params, w0.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemsetAsync(h->m, 0, sizeof(float) * NPARAM, h->s));
     HIPCHK(hipMemsetAsync(h->v, 0, sizeof(float) * NPARAM, h->s));
