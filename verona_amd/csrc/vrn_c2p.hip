@@ -118,7 +118,7 @@ __device__ __forceinline__ Feat c4_features(double D, double Qt, double S2)
 __device__ __forceinline__ bool c4_targets(const double *P, double g, const Feat &f, float *t)
 {
     const double p = (g - 1.0) * P[1];
-    const double gam = __builtin_sqrt(1.0 + P[2] * P[2] + P[3] * P[3] + P[4] * P[4]);
+    const double gam = vrn_sqrt1(1.0 + P[2] * P[2] + P[3] * P[3] + P[4] * P[4]);
     const double q = log1p(p / f.E);
     const double t0 = log(q / f.z), t1 = log(gam);
     t[0] = (float)t0;
@@ -147,7 +147,7 @@ __device__ __forceinline__ bool newton_1dw_rel(double D, double Qt, double S2, d
 {
     const double c = (g - 1.0) / g;
     const double W_max = 1e30;
-    const double W_min = __builtin_sqrt(S2) * (1.0 + 1e-15);
+    const double W_min = vrn_sqrt(S2) * (1.0 + 1e-15);
     for (int t10 = 0; t10 < 1100 && S2 / (W * W) >= 1.0 && W < W_max; t10++) W *= 10.0;
     for (int it = 0; it < n_iter; it++) {
         iters++;
@@ -157,7 +157,7 @@ __device__ __forceinline__ bool newton_1dw_rel(double D, double Qt, double S2, d
             v2 = 0.0;
         else if (v2 > 1.0 - 1e-15)
             v2 = 1.0 - 1e-15;
-        const double sv = __builtin_sqrt(1.0 - v2);
+        const double sv = vrn_sqrt(1.0 - v2);
         const double fun = Qt + W - c * (W * (1.0 - v2) - D * sv);
         const double W3 = W * W * W;
         const double jac = 1.0 - c * (1.0 - v2) - c * (W * (2.0 * S2 / W3) - D * (S2 / (W3 * sv)));
@@ -176,7 +176,7 @@ __device__ __forceinline__ void recover(double D, double Qx, double Qy, double Q
                                         double g, double *P)
 {
     const double v2 = S2 / (W * W);
-    const double gam = 1.0 / __builtin_sqrt(1.0 - v2);
+    const double gam = 1.0 / vrn_sqrt(1.0 - v2);
     P[0] = jmax(1e-8, D / gam);
     P[1] = jmax(1e-8, (W / (gam * gam) - D / gam) / g);
     P[2] = gam * Qx / W;
@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(256, 3) k_solve(SolveArgs a)
         }
         if constexpr (GUESS == VRN_GUESS_GENERIC || GUESS == VRN_GUESS_WARM) {
             // W from the guess as UtoP forms it (src/3dim/structs.jl:276-287)
-            const double gm = __builtin_sqrt(Pg[2] * Pg[2] + Pg[3] * Pg[3] + Pg[4] * Pg[4] + 1.0);
+            const double gm = vrn_sqrt1(Pg[2] * Pg[2] + Pg[3] * Pg[3] + Pg[4] * Pg[4] + 1.0);
             W = (Pg[0] + a.g * Pg[1]) * (gm * gm);
         } else if constexpr (GUESS == VRN_GUESS_ANALYTIC) {
             W = -u[1];

@@ -9,6 +9,8 @@
 
 #include <type_traits>
 
+#include "vrn_sqrt.hpp"
+
 // VRN_NS: namespace of this instantiation.  With VRN_FAST the same functions
 // are built as the fast FP64 form: a*b+c contracted to FMA (the including
 // file sets the contraction pragma) and fdiv() = reciprocal refined by two
@@ -65,7 +67,15 @@ __host__ __device__ __forceinline__ double fdiv(double a, double b)
 #else
 __host__ __device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
 #endif
-__host__ __device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
+// FP32 form: reciprocal instruction (about 1 ulp) times the numerator.
+__host__ __device__ __forceinline__ float fdiv(float a, float b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * __builtin_amdgcn_rcpf(b);
+#else
+    return a / b;
+#endif
+}
 
 enum { REC_MINMOD = 0, REC_PPM = 1, REC_WENOZ = 2 };
 enum { RIEM_HLL = 0, RIEM_HLLC = 1 };
@@ -241,9 +251,9 @@ __device__ __forceinline__ void recon(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T 
 template <int NV, class T> __host__ __device__ __forceinline__ T lorentz(const T *P)
 {
     if constexpr (NV == 5)
-        return dsqrt(P[2] * P[2] + P[3] * P[3] + P[4] * P[4] + T(1));
+        return vrn_sqrt1(P[2] * P[2] + P[3] * P[3] + P[4] * P[4] + T(1));
     else
-        return dsqrt(P[2] * P[2] + P[3] * P[3] + T(1));
+        return vrn_sqrt1(P[2] * P[2] + P[3] * P[3] + T(1));
 }
 
 template <int NV, class T> __device__ __forceinline__ void p2u(const T *P, T *U, T g, T gam)
@@ -285,7 +295,7 @@ __device__ __forceinline__ void p2f(const T *P, T *F, T g, T gam)
 // eos.jl:15-17
 template <class T> __host__ __device__ __forceinline__ T sound_speed(T rho, T u, T g)
 {
-    return dsqrt(fdiv(g * (g - T(1)) * u, rho + g * u));
+    return vrn_sqrt(fdiv(g * (g - T(1)) * u, rho + g * u));
 }
 
 // 3dim/algos.jl:146-157: sigma = c^2/(lor^2 (1-c^2)),
@@ -294,7 +304,7 @@ template <class T>
 __host__ __device__ __forceinline__ void lambdas(T v, T cs, T lor2, T &lp, T &lm)
 {
     T sig = fdiv(cs * cs, lor2 * (T(1) - cs * cs));
-    T rt = dsqrt(sig * (T(1) - v * v + sig));
+    T rt = vrn_sqrt(sig * (T(1) - v * v + sig));
 #if defined(VRN_FAST)
     T inv = fdiv(T(1), T(1) + sig);
     lp = (v + rt) * inv;
@@ -325,7 +335,7 @@ __device__ __forceinline__ void side_speeds(const double *P, double lor, double 
     v = P[2 + DIM] * (d * s * r);
     const double sig = a * (lor * s * r);
     const double inv = d * (lor * d * r);
-    const double rt = dsqrt(sig * (1.0 - v * v + sig));
+    const double rt = vrn_sqrt(sig * (1.0 - v * v + sig));
     lp = (v + rt) * inv;
     lm = (v - rt) * inv;
 }
@@ -359,7 +369,7 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
         bool ok = disc >= T(0);
         T lam = T(0), ps = T(0);
         if (ok) {
-            T qq = b + dsqrt(disc);
+            T qq = b + vrn_sqrt(disc);
             lam = fdiv(T(2) * m_h, qq);
             ps = Fm_h - lam * FE_h;
             ok = __builtin_isfinite(lam) && lam > SL && lam < SR && ps > T(0);
@@ -454,7 +464,7 @@ __host__ __device__ __forceinline__ double cell_rate(const double *P, double g, 
 #pragma unroll
     for (int d = 0; d < NV - 2; d++) {
         const double v = P[2 + d] * ilor;
-        const double rt = dsqrt(sig * (1.0 - v * v + sig));
+        const double rt = vrn_sqrt(sig * (1.0 - v * v + sig));
         r = r + ((dabs(v) + rt) * inv) * inv_dx[d];
     }
 #else
@@ -479,10 +489,10 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
     const double D = U[0], Qt = U[1], Qx = U[2], Qy = U[3], Qz = U[4];
     const double c = (g - 1.0) / g;
     double S2 = Qx * Qx + Qy * Qy + Qz * Qz;
-    double gm = dsqrt(Pg[2] * Pg[2] + Pg[3] * Pg[3] + Pg[4] * Pg[4] + 1.0);
+    double gm = vrn_sqrt1(Pg[2] * Pg[2] + Pg[3] * Pg[3] + Pg[4] * Pg[4] + 1.0);
     double W = (Pg[0] + g * Pg[1]) * (gm * gm);
     const double W_max = 1e30;
-    double W_min = dsqrt(S2) * (1.0 + 1e-15);
+    double W_min = vrn_sqrt(S2) * (1.0 + 1e-15);
     for (int t10 = 0; t10 < 1100 && S2 / (W * W) >= 1.0 && W < W_max; t10++) W *= 10.0; /* unbounded in the reference; W = 0 would never exit */
     bool conv = false;
     for (int it = 0; it < n_iter; it++) {
@@ -493,7 +503,7 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
             v2 = 0.0;
         else if (v2 > 1.0 - 1e-15)
             v2 = 1.0 - 1e-15;
-        double sv = dsqrt(1.0 - v2);
+        double sv = vrn_sqrt(1.0 - v2);
         double fun = Qt + W - c * (W * (1.0 - v2) - D * sv);
         double W3 = W * W * W;
 #if defined(VRN_FAST)
@@ -514,7 +524,7 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
     }
     if (!conv) return false;
     double v2 = fdiv(S2, W * W);
-    double gam = fdiv(1.0, dsqrt(1.0 - v2));
+    double gam = fdiv(1.0, vrn_sqrt(1.0 - v2));
     gam = jmin(gam, 50.0);
     P[0] = jmax(1e-8, fdiv(D, gam));
     P[1] = jmax(1e-8, fdiv(fdiv(W, gam * gam) - fdiv(D, gam), g));

@@ -1,0 +1,64 @@
+// vrn_sqrt.hpp -- FP64 square root for the hot paths, built on v_rsq_f64.
+//
+// sqrt_core(x): y = rsq(x), one Goldschmidt step for g ~ sqrt(x) and
+// h ~ 1/(2 sqrt(x)), then two Newton corrections g += (x - g^2) h.  For
+// x >= 2^-767 this is exactly the arithmetic of the compiler's correctly
+// rounded sqrt() on gfx950 (LLVM's f64 sqrt lowering), which additionally
+// rescales arguments below 2^-767 by 2^256 and patches +-0 and +inf with a
+// class test: 18 instructions against 10 (13 with the patch).  Results are
+// therefore identical wherever the argument is 0, +inf, NaN, negative or at
+// least 2^-767 (about 1.6e-231).  Below that the missing rescaling loses
+// accuracy; the call sites only pass quantities bounded far away from it
+// (1 + |u|^2, sound speeds and wave-speed discriminants of floored states,
+// 1 - v^2 >= 1e-15, |S|^2 of non-vacuum momenta).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double vrn_sqrt_core(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+    g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+    return g;
+}
+#endif
+
+// sqrt(x) for x = +-0, +inf, NaN, x < 0 or x >= 2^-767.
+__host__ __device__ __forceinline__ double vrn_sqrt(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double g = vrn_sqrt_core(x);
+    return __builtin_amdgcn_class(x, 0x260) ? x : g;  // 0x260: -0, +0, +inf
+#else
+    return __builtin_sqrt(x);
+#endif
+}
+
+// sqrt(x) for finite x >= 1 (Lorentz factors sqrt(1 + |u|^2)) or NaN.
+__host__ __device__ __forceinline__ double vrn_sqrt1(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return vrn_sqrt_core(x);
+#else
+    return __builtin_sqrt(x);
+#endif
+}
+
+// FP32 (the FP32 reconstruction/Riemann form): the hardware instruction,
+// about 1 ulp, on the device.
+__host__ __device__ __forceinline__ float vrn_sqrt(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
+__host__ __device__ __forceinline__ float vrn_sqrt1(float x) { return vrn_sqrt(x); }

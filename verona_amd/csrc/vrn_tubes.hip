@@ -34,7 +34,7 @@ namespace vrn {
 // 1dim/structs.jl:58-76  PtoU / PtoF (w = (u + p) + rho association)
 __device__ __forceinline__ void p2u1(const double *P, double *U, double g)
 {
-    double gamma = dsqrt(P[2] * P[2] + 1.0);
+    double gamma = vrn_sqrt1(P[2] * P[2] + 1.0);
     double pr = (g - 1.0) * P[1];
     U[0] = P[0] * gamma;
     U[1] = -(P[1] + pr + P[0]) * (gamma * gamma) + pr;
@@ -43,7 +43,7 @@ __device__ __forceinline__ void p2u1(const double *P, double *U, double g)
 
 __device__ __forceinline__ void p2f1(const double *P, double *F, double g)
 {
-    double gamma = dsqrt(P[2] * P[2] + 1.0);
+    double gamma = vrn_sqrt1(P[2] * P[2] + 1.0);
     double pr = (g - 1.0) * P[1];
     F[0] = P[0] * P[2];
     F[1] = -(P[1] + pr + P[0]) * gamma * P[2];
@@ -59,8 +59,8 @@ __device__ __forceinline__ void face1(const double *PL, const double *PR, double
     p2u1(PL, UL, g);
     p2f1(PR, FR, g);
     p2f1(PL, FL, g);
-    double vL = PL[2] / dsqrt(PL[2] * PL[2] + 1.0);
-    double vR = PR[2] / dsqrt(PR[2] * PR[2] + 1.0);
+    double vL = PL[2] / vrn_sqrt1(PL[2] * PL[2] + 1.0);
+    double vR = PR[2] / vrn_sqrt1(PR[2] * PR[2] + 1.0);
     double CL = sound_speed(PL[0], PL[1], g);
     double CR = sound_speed(PR[0], PR[1], g);
     double lpL, lmL, lpR, lmR;
@@ -81,12 +81,12 @@ __device__ __forceinline__ void c2p1(const double *U, double *x, double g, int n
         // once all three unknowns are NaN every further iteration returns NaN:
         // stopping here gives the same result as running out the iterations
         if (__builtin_isnan(x[0]) && __builtin_isnan(x[1]) && __builtin_isnan(x[2])) break;
-        double gam = dsqrt(x[2] * x[2] + 1.0);
+        double gam = vrn_sqrt1(x[2] * x[2] + 1.0);
         double w = g * x[1] + x[0];
         double f0 = gam * x[0] - U[0];
         double f1 = (g - 1.0) * x[1] - gam * gam * w - U[1];
         double f2 = x[2] * gam * w - U[2];
-        double gj = dsqrt(1.0 + x[2] * x[2]);
+        double gj = vrn_sqrt1(1.0 + x[2] * x[2]);
         // column-major a[0..8]
         double a0 = gj, a1 = -(gj * gj), a2 = gj * x[2];
         double a3 = 0.0, a4 = (g - 1.0) - gj * gj * g, a5 = gj * x[2] * g;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void c2p1(const double *U, double *x, double g, int n
         double d0 = ((a4 * a8 - a7 * a5) * f0 + (a6 * a5 - a3 * a8) * f1 + (a3 * a7 - a6 * a4) * f2) / d;
         double d1 = ((a7 * a2 - a1 * a8) * f0 + (a0 * a8 - a6 * a2) * f1 + (a6 * a1 - a0 * a7) * f2) / d;
         double d2 = ((a1 * a5 - a4 * a2) * f0 + (a3 * a2 - a0 * a5) * f1 + (a0 * a4 - a3 * a1) * f2) / d;
-        if (dsqrt(d0 * d0 + d1 * d1 + d2 * d2) < tol) break;
+        if (vrn_sqrt(d0 * d0 + d1 * d1 + d2 * d2) < tol) break;
         x[0] = x[0] - d0;
         x[1] = x[1] - d1;
         x[2] = x[2] - d2;
