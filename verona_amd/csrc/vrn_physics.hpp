@@ -67,11 +67,16 @@ __host__ __device__ __forceinline__ double fdiv(double a, double b)
 #else
 __host__ __device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
 #endif
-// FP32 form: reciprocal instruction (about 1 ulp) times the numerator.
+// FP32 form: reciprocal instruction times the numerator plus one residual
+// correction, so quotients that are representable come out exact (WENO-Z
+// returns constant data unchanged) at 4 instructions instead of IEEE
+// division's 10.
 __host__ __device__ __forceinline__ float fdiv(float a, float b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return a * __builtin_amdgcn_rcpf(b);
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = a * r;
+    return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
 #else
     return a / b;
 #endif
