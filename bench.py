@@ -315,6 +315,8 @@ def main():
     ap.add_argument("--c2p-only", action="store_true", help="only the learned con2prim leg")
     ap.add_argument("--c2p-quick", action="store_true",
                     help="with --c2p-only: the short profiling sequence (800 training steps, 2 solves per guess)")
+    ap.add_argument("--tubes-only", action="store_true", help="only the batched 1D tubes leg")
+    ap.add_argument("--tubes-steps", type=int, default=500, help="steps of the batched tubes leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of one 384^3 slot (known bytes) for counter calibration")
     ap.add_argument("--precision", default="fp64fast", choices=["fp64", "fp64fast"],
@@ -338,6 +340,9 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if args.c2p_only:
         print(json.dumps({"con2prim": run_c2p(log=log, quick=args.c2p_quick)}), flush=True)
+        return
+    if args.tubes_only:
+        print(json.dumps({"batched_tubes": run_tubes(steps=args.tubes_steps)}), flush=True)
         return
     cells = n ** 3
     el, rep, nbad, nresc = run_gpu(n, K, W, ref=False, precision=args.precision)
