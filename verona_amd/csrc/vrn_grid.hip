@@ -153,8 +153,8 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a)
 #pragma unroll
         for (int v = 0; v < NV; v++) p[v] = a.Pg[v * ps + src];
         if (a.do_limit) {
-            p[0] = jmax(p[0], a.floor_);
-            p[1] = jmax(p[1], a.floor_);
+            p[0] = floor_max(a.floor_, p[0]);
+            p[1] = floor_max(a.floor_, p[1]);
         }
         if (src != dst || a.do_limit || a.Pd != a.Pg) {
 #pragma unroll
@@ -923,6 +923,10 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     if (!(p->dx > 0) || !(p->dy > 0) || (c->ndim == 3 && !(p->dz > 0)))
         return fail(VRN_E_ARG, "grid spacings must be positive");
     if (!(p->cfl > 0) && !(p->dt > 0)) return fail(VRN_E_ARG, "need dt > 0 or cfl > 0");
+    // floors are applied as max(floor, x) by a compare-select, which needs a
+    // floor that is a number and not -0
+    if (std::isnan(p->floor_) || (p->floor_ == 0.0 && std::signbit(p->floor_)))
+        return fail(VRN_E_ARG, "floor must be a number other than -0");
     for (int f = 0; f < 6; f++)
         if (p->bc[f] < 0 || p->bc[f] > 3 || (p->bc[f] == VRN_BC_NOZZLE && (f != 4 || !c->have_noz)))
             return fail(VRN_E_ARG, "bad boundary kind %d on face %d (nozzle needs vrn_set_nozzle)", p->bc[f], f);

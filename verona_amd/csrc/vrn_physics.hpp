@@ -36,6 +36,11 @@ template <class T> __host__ __device__ __forceinline__ T jmin(T x, T y)
     T am = __builtin_signbit(d) ? x : y;
     return (__builtin_isnan(x) || __builtin_isnan(y)) ? d : am;
 }
+// Julia max(c, x) / min(x, c) against a bound c that is neither NaN nor -0
+// (floors, caps): one compare and a select.  Same value as jmax/jmin for
+// every x; a NaN x comes back as itself (jmax returns c - x, also NaN).
+template <class T> __host__ __device__ __forceinline__ T floor_max(T c, T x) { return x <= c ? c : x; }
+template <class T> __host__ __device__ __forceinline__ T cap_min(T x, T c) { return x >= c ? c : x; }
 // Julia sign(): -1, +1, or x itself for +-0 and NaN.
 template <class T> __device__ __forceinline__ T jsign(T x)
 {
@@ -530,9 +535,9 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
     if (!conv) return false;
     double v2 = fdiv(S2, W * W);
     double gam = fdiv(1.0, vrn_sqrt(1.0 - v2));
-    gam = jmin(gam, 50.0);
-    P[0] = jmax(1e-8, fdiv(D, gam));
-    P[1] = jmax(1e-8, fdiv(fdiv(W, gam * gam) - fdiv(D, gam), g));
+    gam = cap_min(gam, 50.0);
+    P[0] = floor_max(1e-8, fdiv(D, gam));
+    P[1] = floor_max(1e-8, fdiv(fdiv(W, gam * gam) - fdiv(D, gam), g));
 #if defined(VRN_FAST)
     const double iW = fdiv(gam, W);
     P[2] = Qx * iW;

@@ -112,8 +112,8 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const d
         }
         if (ok) {
             if (!all_finite<NV>(Pn)) atomicAdd(&a.st->n_unrecovered, 1ull);
-            Pn[0] = jmax(Pn[0], a.floor_);
-            Pn[1] = jmax(Pn[1], a.floor_);
+            Pn[0] = floor_max(a.floor_, Pn[0]);
+            Pn[1] = floor_max(a.floor_, Pn[1]);
 #pragma unroll
             for (int v = 0; v < NV; v++) a.Pout[v * ps + cu] = Pn[v];
             if (a.do_rate) rate = fmax(rate, cell_rate<NV>(Pn, a.g, a.inv_dx));
@@ -193,10 +193,10 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
             const double *q = P + v * ps + rbu + c;
             recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
         }
-        lo[0] = jmax(fl, lo[0]);
-        lo[1] = jmax(fl, lo[1]);
-        up[0] = jmax(fl, up[0]);
-        up[1] = jmax(fl, up[1]);
+        lo[0] = floor_max(fl, lo[0]);
+        lo[1] = floor_max(fl, lo[1]);
+        up[0] = floor_max(fl, up[0]);
+        up[1] = floor_max(fl, up[1]);
         R upl[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -325,10 +325,10 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         for (int v = 0; v < NV; v++)
             recon<REC>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
                        wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
-        lo[0] = jmax(fl, lo[0]);
-        lo[1] = jmax(fl, lo[1]);
-        up[0] = jmax(fl, up[0]);
-        up[1] = jmax(fl, up[1]);
+        lo[0] = floor_max(fl, lo[0]);
+        lo[1] = floor_max(fl, lo[1]);
+        up[0] = floor_max(fl, up[0]);
+        up[1] = floor_max(fl, up[1]);
         if (c >= m0) {
             R Ff[NV];
             face_flux<NV, DIM, RIEM>(upp, lo, g, Ff);
@@ -387,8 +387,8 @@ __global__ __launch_bounds__(64) void k_rescue(SweepArgs a)
         int iters = 0;
         c2p_cascade_tail(U, Pn, a.g, a.n_iter, a.tol, iters);
         if (!all_finite<5>(Pn)) atomicAdd(&a.st->n_unrecovered, 1ull);
-        Pn[0] = jmax(Pn[0], a.floor_);
-        Pn[1] = jmax(Pn[1], a.floor_);
+        Pn[0] = floor_max(a.floor_, Pn[0]);
+        Pn[1] = floor_max(a.floor_, Pn[1]);
 #pragma unroll
         for (int v = 0; v < 5; v++) a.Pout[v * ps + cu] = Pn[v];
         if (a.do_rate) rate = fmax(rate, cell_rate<5>(Pn, a.g, a.inv_dx));
