@@ -458,12 +458,12 @@ struct Timed {
         }
         ev = c->ev_free.back();
         c->ev_free.pop_back();
-        hipEventRecord(ev.first, c->s);
+        (void)hipEventRecord(ev.first, c->s);
     }
     ~Timed()
     {
         if (!on) return;
-        hipEventRecord(ev.second, c->s);
+        (void)hipEventRecord(ev.second, c->s);
         c->ev_used.push_back({kn, ev});
     }
 };
@@ -604,7 +604,7 @@ int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
     }
     e = hipMalloc(&c->buf, total * sizeof(double));
     if (e != hipSuccess) {
-        hipStreamDestroy(c->s);
+        (void)hipStreamDestroy(c->s);
         delete c;
         return fail(VRN_E_NOMEM, "hipMalloc(%zu bytes): %s", total * sizeof(double), hipGetErrorString(e));
     }
@@ -634,20 +634,20 @@ int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
 int vrn_destroy(vrn_ctx *c)
 {
     if (!c) return 0;
-    if (c->s) hipStreamSynchronize(c->s);
+    if (c->s) (void)hipStreamSynchronize(c->s);
     for (auto &e : c->ev_used) {
-        hipEventDestroy(e.second.first);
-        hipEventDestroy(e.second.second);
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
     }
     for (auto &e : c->ev_free) {
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
-    if (c->buf) hipFree(c->buf);
-    if (c->rescue) hipFree(c->rescue);
-    if (c->st) hipFree(c->st);
-    if (c->st_host) hipHostFree(c->st_host);
-    if (c->s) hipStreamDestroy(c->s);
+    if (c->buf) (void)hipFree(c->buf);
+    if (c->rescue) (void)hipFree(c->rescue);
+    if (c->st) (void)hipFree(c->st);
+    if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
     return 0;
 }

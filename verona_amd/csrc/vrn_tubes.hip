@@ -370,18 +370,18 @@ int vrn_tubes_create(int ntubes, const int64_t *lengths, double gamma, vrn_tubes
 int vrn_tubes_destroy(vrn_tubes *c)
 {
     if (!c) return 0;
-    if (c->s) hipStreamSynchronize(c->s);
-    if (c->gexec) hipGraphExecDestroy(c->gexec);
-    if (c->graph) hipGraphDestroy(c->graph);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    hipFree(c->buf);
-    hipFree(c->off);
-    hipFree(c->order);
-    hipFree(c->dt);
-    hipFree(c->dx);
-    hipFree(c->queue);
-    if (c->s) hipStreamDestroy(c->s);
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    (void)hipFree(c->buf);
+    (void)hipFree(c->off);
+    (void)hipFree(c->order);
+    (void)hipFree(c->dt);
+    (void)hipFree(c->dx);
+    (void)hipFree(c->queue);
+    if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
     return 0;
 }
@@ -460,8 +460,8 @@ int vrn_tubes_run(vrn_tubes *c, int nsteps, int block, int grid, int use_graph, 
         bool stale = !c->gexec || c->g_block != block || c->g_grid != grid || c->g_steps != nsteps ||
                      c->g_rec != c->rec || c->g_riem != c->riemann;
         if (stale) {
-            if (c->gexec) hipGraphExecDestroy(c->gexec);
-            if (c->graph) hipGraphDestroy(c->graph);
+            if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+            if (c->graph) (void)hipGraphDestroy(c->graph);
             c->gexec = nullptr;
             c->graph = nullptr;
             TCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
