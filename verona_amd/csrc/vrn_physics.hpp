@@ -69,6 +69,18 @@ __host__ __device__ __forceinline__ double fdiv(double a, double b)
     return a / b;
 #endif
 }
+// 1/b: reciprocal refined by two Newton steps (about an ulp), for b bounded
+// away from zero and from the subnormal range.
+__host__ __device__ __forceinline__ double frcp(double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+#else
+    return 1.0 / b;
+#endif
+}
 #else
 __host__ __device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
 #endif
@@ -97,7 +109,9 @@ template <> struct WenoEps<double> { static constexpr double v = 1.0e-42; };
 template <> struct WenoEps<float> { static constexpr float v = 1.0e-30f; };
 
 // fluxlimiter.jl:19-71  WENOZ(q_{i-2..i+2}) -> lo = q at i-1/2, hi = q at i+1/2
-template <class T>
+// SHARED_RCP (fast FP64 form): both faces' normalisations from one
+// reciprocal -- fewer instructions, four more live registers per call.
+template <bool SHARED_RCP = false, class T>
 __device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
 {
     const T b1c = T(13.0) / T(12.0), b2c = T(0.25);
@@ -152,14 +166,30 @@ __device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T 
     T f0 = T(2) * qm2 - T(7) * qm1 + T(11) * q;
     T f1 = -qm1 + T(5) * q + T(2) * qp1;
     T f2 = T(2) * q + T(5) * qp1 - qp2;
-    hi = fdiv(f0 * a0 + f1 * a1 + f2 * a2, T(6) * (a0 + a1 + a2));
+    const T nh = f0 * a0 + f1 * a1 + f2 * a2, dh = T(6) * (a0 + a1 + a2);
+#if defined(VRN_FAST)
+    constexpr bool share = SHARED_RCP && std::is_same<T, double>::value;
+#else
+    constexpr bool share = false;
+#endif
+    if constexpr (!share) hi = fdiv(nh, dh);
 
     a0 = T(0.1) * s2;
     a2 = T(0.3) * s0;
     f0 = T(2) * qp2 - T(7) * qp1 + T(11) * q;
     f1 = -qp1 + T(5) * q + T(2) * qm1;
     f2 = T(2) * q + T(5) * qm1 - qm2;
-    lo = fdiv(f0 * a0 + f1 * a1 + f2 * a2, T(6) * (a0 + a1 + a2));
+    const T nl = f0 * a0 + f1 * a1 + f2 * a2, dl = T(6) * (a0 + a1 + a2);
+    if constexpr (share) {
+        // one reciprocal for both faces: each denominator is >= 6 (every
+        // s >= 1) and their product stays finite unless a stencil jump
+        // exceeds ~1e17
+        const T r = frcp(dh * dl);
+        hi = nh * (dl * r);
+        lo = nl * (dh * r);
+    } else {
+        lo = fdiv(nl, dl);
+    }
 }
 
 // fluxlimiter.jl:1-13  MINMOD(q_{i-1}, q_i, q_{i+1})
@@ -243,11 +273,11 @@ __device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T
     hi = ql_ip1;
 }
 
-template <int REC, class T>
+template <int REC, bool SHARED_RCP = false, class T>
 __device__ __forceinline__ void recon(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
 {
     if constexpr (REC == REC_WENOZ)
-        wenoz(qm2, qm1, q, qp1, qp2, lo, hi);
+        wenoz<SHARED_RCP>(qm2, qm1, q, qp1, qp2, lo, hi);
     else if constexpr (REC == REC_PPM)
         ppm(qm2, qm1, q, qp1, qp2, lo, hi);
     else
@@ -534,16 +564,23 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
     }
     if (!conv) return false;
     double v2 = fdiv(S2, W * W);
-    double gam = fdiv(1.0, vrn_sqrt(1.0 - v2));
-    gam = cap_min(gam, 50.0);
-    P[0] = floor_max(1e-8, fdiv(D, gam));
-    P[1] = floor_max(1e-8, fdiv(fdiv(W, gam * gam) - fdiv(D, gam), g));
 #if defined(VRN_FAST)
-    const double iW = fdiv(gam, W);
+    // in t = sqrt(1 - v^2) = 1/Gamma, capped below at 1/50 (Gamma <= 50):
+    // D/Gamma = D t, W/Gamma^2 - D/Gamma = (W t - D) t, Gamma Q/W = Q/(W t);
+    // a NaN t (v^2 > 1) propagates as in the reference
+    double t = vrn_sqrt(1.0 - v2);
+    t = t < 0.02 ? 0.02 : t;
+    P[0] = floor_max(1e-8, D * t);
+    P[1] = floor_max(1e-8, (W * t - D) * t * frcp(g));
+    const double iW = frcp(W * t);
     P[2] = Qx * iW;
     P[3] = Qy * iW;
     P[4] = Qz * iW;
 #else
+    double gam = fdiv(1.0, vrn_sqrt(1.0 - v2));
+    gam = cap_min(gam, 50.0);
+    P[0] = floor_max(1e-8, fdiv(D, gam));
+    P[1] = floor_max(1e-8, fdiv(fdiv(W, gam * gam) - fdiv(D, gam), g));
     P[2] = gam * Qx / W;
     P[3] = gam * Qy / W;
     P[4] = gam * Qz / W;

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             const double *q = P + v * ps + rbu + c;
-            recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
+            recon<REC, true>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
         }
         lo[0] = floor_max(fl, lo[0]);
         lo[1] = floor_max(fl, lo[1]);
@@ -323,7 +323,8 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         R lo[NV], up[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++)
-            recon<REC>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
+            // the last sweep (RK combine + con2prim) has no registers to spare
+            recon<REC, !LAST>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
                        wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
         lo[0] = floor_max(fl, lo[0]);
         lo[1] = floor_max(fl, lo[1]);
