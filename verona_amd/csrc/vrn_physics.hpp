@@ -99,6 +99,15 @@ __host__ __device__ __forceinline__ float fdiv(float a, float b)
 #endif
 }
 
+// 1/x: refined reciprocal in the fast FP64 form, IEEE division otherwise
+// (FP32: the reciprocal-plus-correction of fdiv).
+#if defined(VRN_FAST)
+__host__ __device__ __forceinline__ double recip(double x) { return frcp(x); }
+#else
+__host__ __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+#endif
+__host__ __device__ __forceinline__ float recip(float x) { return fdiv(1.0f, x); }
+
 enum { REC_MINMOD = 0, REC_PPM = 1, REC_WENOZ = 2 };
 enum { RIEM_HLL = 0, RIEM_HLLC = 1 };
 
@@ -371,7 +380,7 @@ __device__ __forceinline__ void side_speeds(const double *P, double lor, double 
     const double b = P[0] + g * (2.0 - g) * P[1];
     const double d = lor * lor * b;
     const double s = d + a;
-    const double r = fdiv(1.0, lor * d * s);
+    const double r = recip(lor * d * s);
     v = P[2 + DIM] * (d * s * r);
     const double sig = a * (lor * s * r);
     const double inv = d * (lor * d * r);
@@ -399,7 +408,7 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
     }
     if constexpr (RIEM == RIEM_HLLC) {
         T SL = -Cmin, SR = Cmax;
-        T inv = fdiv(T(1), Cmax + Cmin);
+        T inv = recip(Cmax + Cmin);
         T E_h = -(Cmax * UR[1] + Cmin * UL[1] + FL[1] - FR[1]) * inv;
         T m_h = (Cmax * UR[nd] + Cmin * UL[nd] + FL[nd] - FR[nd]) * inv;
         T FE_h = -(FR[1] * Cmin + FL[1] * Cmax - Cmax * Cmin * (UR[1] - UL[1])) * inv;
@@ -419,7 +428,7 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
             T S = left ? SL : SR;
             T vK = left ? vL : vR;
             T pK = left ? pL : pR;
-            T fac = fdiv(T(1), S - lam);
+            T fac = recip(S - lam);
             T sv = S - vK;
             T Us[NV];
             T UK[NV], FK[NV];
@@ -442,7 +451,7 @@ __device__ __forceinline__ void riemann_solve(const T *UL, const T *UR, const T 
         }
     }
 #if defined(VRN_FAST)
-    const T invs = fdiv(T(1), Cmax + Cmin);
+    const T invs = recip(Cmax + Cmin);
 #pragma unroll
     for (int v = 0; v < NV; v++)
         F[v] = (FR[v] * Cmin + FL[v] * Cmax - Cmax * Cmin * (UR[v] - UL[v])) * invs;
@@ -499,7 +508,7 @@ __host__ __device__ __forceinline__ double cell_rate(const double *P, double g, 
     const double b = P[0] + g * (2.0 - g) * P[1];
     const double dd = lor * lor * b;
     const double s = dd + a;
-    const double rr = fdiv(1.0, lor * dd * s);
+    const double rr = recip(lor * dd * s);
     const double ilor = dd * s * rr, sig = a * (lor * s * rr), inv = dd * (lor * dd * rr);
 #pragma unroll
     for (int d = 0; d < NV - 2; d++) {
