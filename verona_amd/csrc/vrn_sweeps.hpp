@@ -131,8 +131,17 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const d
 // c = x0-1 .. x1 (their reconstructions); the lane with cell c also solves
 // face c-1/2 (c >= x0) and updates cell c-1 (c >= x0+1).  Whole rows are
 // x0 = 3, x1 = NX-3.
+// Waves per SIMD the x sweep is compiled for: the fast FP64 WENO-Z
+// instantiations fit 128 VGPRs without spilling and run ~3 % faster at 4 than
+// at the unbounded 3; the others would spill, so they stay unbounded.
+#if defined(VRN_FAST)
+template <class R, int REC> constexpr int xsweep_waves() { return REC == REC_WENOZ ? 4 : 1; }
+#else
+template <class R, int REC> constexpr int xsweep_waves() { return 1; }
+#endif
+
 template <int NV, class R, int REC, int RIEM, bool LAST>
-__global__ __launch_bounds__(256) void k_xsweep(SweepArgs a)
+__global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepArgs a)
 {
     const Geo G = a.geo;
     const int lane = threadIdx.x & 63;
