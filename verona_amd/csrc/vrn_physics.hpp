@@ -13,8 +13,8 @@
 
 // VRN_NS: namespace of this instantiation.  With VRN_FAST the same functions
 // are built as the fast FP64 form: a*b+c contracted to FMA (the including
-// file sets the contraction pragma) and fdiv() = reciprocal refined by two
-// Newton steps times the numerator with one residual correction (within an
+// file sets the contraction pragma) and fdiv() = reciprocal refined by one
+// Newton step times the numerator with one residual correction (within an
 // ulp of the correctly rounded quotient), used only where the divisor is
 // bounded away from zero and from the subnormal range.
 #ifndef VRN_NS
@@ -57,11 +57,11 @@ __host__ __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf
 __host__ __device__ __forceinline__ double fdiv(double a, double b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
+    // v_rcp_f64 is good to ~2^-23; one Newton step leaves |1 - b r| ~ 2^-46,
+    // and the residual correction of the quotient squares that again, so
+    // the result is within an ulp (a second step on r would change nothing)
     double r = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
     double q = a * r;
     double rem = __builtin_fma(-b, q, a);
     return __builtin_fma(rem, r, q);
@@ -542,7 +542,14 @@ __device__ __forceinline__ bool c2p_1dw(const double *U, const double *Pg, doubl
     double W = (Pg[0] + g * Pg[1]) * (gm * gm);
     const double W_max = 1e30;
     double W_min = vrn_sqrt(S2) * (1.0 + 1e-15);
+#if defined(VRN_FAST)
+    // S2 / W^2 >= 1 without the division: equal except where the quotient
+    // rounds across 1 (then only the starting W differs by the factor 10);
+    // S2 = W = 0 gives NaN >= 1, false, in the reference
+    for (int t10 = 0; t10 < 1100 && S2 >= W * W && S2 > 0.0 && W < W_max; t10++) W *= 10.0;
+#else
     for (int t10 = 0; t10 < 1100 && S2 / (W * W) >= 1.0 && W < W_max; t10++) W *= 10.0; /* unbounded in the reference; W = 0 would never exit */
+#endif
     bool conv = false;
     for (int it = 0; it < n_iter; it++) {
         iters++;
