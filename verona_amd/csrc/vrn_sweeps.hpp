@@ -16,21 +16,22 @@ using vrn::gidx;
 using vrn::XOFF;
 
 // ------------------------------------------------------------ wave helpers
-// Value of lane l-1 (lane 0 gets garbage and is overridden by the caller):
-// DPP wave_shr:1 (GFX9 wave-wide shift), a VALU modifier -- no LDS round trip.
-__device__ __forceinline__ int dpp_shr1(int x)
+// Value of lane l-1, and `fill` in lane 0: DPP wave_shr:1 (GFX9 wave-wide
+// shift, a VALU operand modifier -- no LDS round trip) with bound_ctrl off,
+// so lane 0, whose source lane does not exist, keeps the old value `fill`.
+__device__ __forceinline__ int dpp_shr1(int x, int fill)
 {
-    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(fill, x, 0x138, 0xf, 0xf, false);
 }
-__device__ __forceinline__ float shift_up1(float v)
+__device__ __forceinline__ float shift_up1(float v, float fill)
 {
-    return __int_as_float(dpp_shr1(__float_as_int(v)));
+    return __int_as_float(dpp_shr1(__float_as_int(v), __float_as_int(fill)));
 }
-__device__ __forceinline__ double shift_up1(double v)
+__device__ __forceinline__ double shift_up1(double v, double fill)
 {
-    long long b = __double_as_longlong(v);
-    int lo = dpp_shr1((int)(b & 0xffffffffll));
-    int hi = dpp_shr1((int)(b >> 32));
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = dpp_shr1((int)(b & 0xffffffffll), (int)(f & 0xffffffffll));
+    const int hi = dpp_shr1((int)(b >> 32), (int)(f >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 // Value of lane 63 in every lane (v_readlane into a scalar register).
@@ -208,18 +209,14 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
         up[1] = floor_max(fl, up[1]);
         R upl[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            R t = shift_up1(up[v]);
-            upl[v] = lane == 0 ? cup[v] : t;
-        }
+        for (int v = 0; v < NV; v++) upl[v] = shift_up1(up[v], cup[v]);
         R Ff[NV];
         face_flux<NV, 0, RIEM>(upl, lo, g, Ff);
         double F[NV], Fl[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             F[v] = (double)Ff[v];
-            double t = shift_up1(F[v]);
-            Fl[v] = lane == 0 ? cF[v] : t;
+            Fl[v] = shift_up1(F[v], cF[v]);
         }
         if constexpr (!LAST) {
             if (upd) {
