@@ -395,7 +395,8 @@ struct vrn_ctx {
     int ndim = 3, NV = 5;
     Geo G{};
     double gamma = 4.0 / 3.0;
-    double *buf = nullptr;  // one allocation for all planes
+    double *buf = nullptr;   // P, Pn, U, Uh and the staging buffer
+    double *fbuf = nullptr;  // flux slots, allocated by the first per-kernel call that uses them
     double *P = nullptr, *Pn = nullptr, *U = nullptr, *Uh = nullptr, *F[3] = {nullptr, nullptr, nullptr};
     double *stg = nullptr;
     int *rescue = nullptr;
@@ -467,6 +468,24 @@ struct Timed {
         c->ev_used.push_back({kn, ev});
     }
 };
+
+// The flux slots (zeroed) on first use.
+int ensure_flux(vrn_ctx *c)
+{
+    if (c->fbuf) return 0;
+    const size_t arr = (size_t)c->NV * c->G.ps;
+    const int nf = c->ndim == 3 ? 3 : 2;
+    hipError_t e = hipMalloc(&c->fbuf, sizeof(double) * arr * nf);
+    if (e != hipSuccess) {
+        c->fbuf = nullptr;
+        return fail(VRN_E_NOMEM, "hipMalloc of the flux slots (%zu bytes): %s", sizeof(double) * arr * nf,
+                    hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(c->fbuf, 0, sizeof(double) * arr * nf, c->s);
+    if (e != hipSuccess) return fail(VRN_E_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    for (int d = 0; d < nf; d++) c->F[d] = c->fbuf + d * arr;
+    return 0;
+}
 
 double *slot_ptr(vrn_ctx *c, int slot)
 {
@@ -594,7 +613,10 @@ int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
         return fail(VRN_E_ARG, "grid too large for 32-bit rescue indices");
     }
     const int NV = c->NV;
-    int nplanes_arrays = ndim == 3 ? 7 : 6;  // P, Pn, U, Uh, Fx, Fy, (Fz)
+    // P, Pn, U, Uh; the flux slots of the per-kernel interface (vrn_fluxes /
+    // vrn_update) are allocated on their first use: the fused step never
+    // touches them, so a step-only context holds 4 arrays + staging
+    const int nplanes_arrays = 4;
     size_t arr = (size_t)NV * c->G.ps;
     size_t total = arr * nplanes_arrays + (size_t)NV * cells;
     hipError_t e = hipStreamCreate(&c->s);
@@ -613,9 +635,6 @@ int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
     c->Pn = p; p += arr;
     c->U = p; p += arr;
     c->Uh = p; p += arr;
-    c->F[0] = p; p += arr;
-    c->F[1] = p; p += arr;
-    if (ndim == 3) { c->F[2] = p; p += arr; }
     c->stg = p;
     e = hipMalloc(&c->rescue, sizeof(int) * (size_t)cells);
     if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(DevState));
@@ -644,6 +663,7 @@ int vrn_destroy(vrn_ctx *c)
         (void)hipEventDestroy(e.second);
     }
     if (c->buf) (void)hipFree(c->buf);
+    if (c->fbuf) (void)hipFree(c->fbuf);
     if (c->rescue) (void)hipFree(c->rescue);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
@@ -662,6 +682,8 @@ int vrn_sync(vrn_ctx *c)
 int vrn_upload(vrn_ctx *c, int slot, const double *host)
 {
     if (int r = check_ctx(c)) return r;
+    if (slot >= VRN_SLOT_FX && slot <= VRN_SLOT_FZ)
+        if (int r = ensure_flux(c)) return r;
     double *d = slot_ptr(c, slot);
     if (!d || !host) return fail(VRN_E_ARG, "bad slot %d or null host pointer", slot);
     const Geo &G = c->G;
@@ -681,6 +703,8 @@ int vrn_upload(vrn_ctx *c, int slot, const double *host)
 int vrn_download(vrn_ctx *c, int slot, double *host)
 {
     if (int r = check_ctx(c)) return r;
+    if (slot >= VRN_SLOT_FX && slot <= VRN_SLOT_FZ)
+        if (int r = ensure_flux(c)) return r;
     double *d = slot_ptr(c, slot);
     if (!d || !host) return fail(VRN_E_ARG, "bad slot %d or null host pointer", slot);
     const Geo &G = c->G;
@@ -698,6 +722,8 @@ int vrn_download(vrn_ctx *c, int slot, double *host)
 int vrn_copy(vrn_ctx *c, int dst_slot, int src_slot)
 {
     if (int r = check_ctx(c)) return r;
+    if ((dst_slot >= VRN_SLOT_FX && dst_slot <= VRN_SLOT_FZ) || (src_slot >= VRN_SLOT_FX && src_slot <= VRN_SLOT_FZ))
+        if (int r = ensure_flux(c)) return r;
     double *d = slot_ptr(c, dst_slot), *s_ = slot_ptr(c, src_slot);
     if (!d || !s_ || d == s_) return fail(VRN_E_ARG, "bad slots %d <- %d", dst_slot, src_slot);
     HIPCHK(hipMemcpyAsync(d, s_, sizeof(double) * (size_t)c->NV * c->G.ps, hipMemcpyDeviceToDevice, c->s));
@@ -764,6 +790,7 @@ int vrn_fluxes(vrn_ctx *c, int dim, double floor_, int rec, int riemann)
     if (int r = check_ctx(c)) return r;
     if (dim < 0 || dim >= c->ndim) return fail(VRN_E_ARG, "dim %d out of range", dim);
     if (rec < 0 || rec > 2 || riemann < 0 || riemann > 1) return fail(VRN_E_ARG, "bad rec/riemann");
+    if (int r = ensure_flux(c)) return r;
     Timed tm(c, KN_FLUX);
     if (c->NV == 5) {
         if (dim == 0) launch_ref_flux<5, 0>(c, rec, riemann, floor_);
@@ -781,6 +808,7 @@ int vrn_update(vrn_ctx *c, int dst_slot, double dt, double dx, double dy, double
 {
     if (int r = check_ctx(c)) return r;
     if (dst_slot != VRN_SLOT_U && dst_slot != VRN_SLOT_UHALF) return fail(VRN_E_ARG, "dst must be U or UHALF");
+    if (int r = ensure_flux(c)) return r;
     double *dst = slot_ptr(c, dst_slot);
     const Geo &G = c->G;
     long n = (long)(G.NX - 6) * (G.NY - 6) * (c->NV == 5 ? G.NZ - 6 : 1);
