@@ -153,3 +153,21 @@ def test_partition_changes_no_arithmetic():
                 dev.step()
             outs.append(dev.download("P"))
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("floor", [float("nan"), -0.0])
+def test_step_rejects_floor_without_a_numeric_bound(floor):
+    """Floors are applied as max(floor, x) by a compare-select, which equals
+    Julia's max only for a numeric bound other than -0: vrn_step refuses the
+    rest with an error instead of producing a different state."""
+    from verona_amd import DeviceGrid
+    from verona_amd._lib import VeronaHIPError
+    progress(f"edge: floor {floor!r} rejected")
+    with DeviceGrid(3, 8, 8, 8, G) as dev:
+        dev.upload("P", state3(8, 8, 8, seed=28))
+        dev.prim_to_cons()
+        bc = ("periodic",) * 6
+        dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision="fp64fast",
+                     n_iter=50, tol=1e-10, floor=floor, cfl=0.4, dx=1 / 8, dy=1 / 8, dz=1 / 8, bc=bc)
+        with pytest.raises(VeronaHIPError, match="floor"):
+            dev.step()
