@@ -392,9 +392,13 @@ def main():
         out["other_fp64_form"] = {"form": other, "value": cells * max(K // 2, 4) / el1,
                                   "unit": "zone-updates/s", "n_unrecovered": nb1}
         el2, rep2, nb2, _ = run_gpu(n, max(K // 2, 4), 2, ref=True, timing=False)
-        out["ref_form"] = {"what": "Verona semantics: WENOZ+HLL, midpoint RK2, fixed dt",
+        out["ref_form"] = {"what": "Verona semantics: WENOZ+HLL, midpoint RK2, fixed dt", "form": "fp64fast",
                            "value": cells * max(K // 2, 4) / el2, "unit": "zone-updates/s",
                            "n_unrecovered": nb2}
+        el2x, _, nb2x, _ = run_gpu(n, max(K // 2, 4), 2, ref=True, precision="fp64", timing=False)
+        out["ref_form_exact"] = {"what": "Verona semantics, FP64 bit-identical to the oracle", "form": "fp64",
+                                 "value": cells * max(K // 2, 4) / el2x, "unit": "zone-updates/s",
+                                 "n_unrecovered": nb2x}
         el3, _, nb3, _ = run_gpu(n, max(K // 2, 4), 2, ref=False, precision="fp32", timing=False)
         out["fp32_form"] = {"what": "reconstruction + Riemann in FP32, con2prim + accumulation FP64",
                             "value": cells * max(K // 2, 4) / el3, "unit": "zone-updates/s",
