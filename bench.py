@@ -172,35 +172,58 @@ def fp32_error(n, steps):
             "rel_l1_per_var": dict(zip(["rho", "u", "u^x", "u^y", "u^z"], err))}
 
 
-def cpu_baseline(budget_s=15.0, n=128):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=12.0, budget_ref_s=6.0, n=128):
     """CPU oracle (plain-C restatement of Verona's algorithm, OpenMP) on a
-    bounded sample of the same workload: an n^3 grid, SSP-RK3 steps until
-    about budget_s seconds of wall time have elapsed."""
+    bounded sample of the same workload: an n^3 grid, steps until about
+    budget_s seconds of wall time have elapsed -- the benchmark's method
+    (WENOZ + HLLC + SSP-RK3, CFL), then Verona's own (WENOZ + HLL, midpoint RK2,
+    fixed dt) for budget_ref_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    P = jet_state(n)
-    Po = np.ascontiguousarray(P.T)
     dx = 1.0 / n
     nz = nozzle(n)
-    mode = O.make_mode(ref=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50,
-                       tol=1e-10, gamma=GAMMA, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx)
     phis = phi_stream(4096)
     bc, keep = O.make_bc3([2, 2, 2, 2, 3, 2], dx=dx, dy=dx,
                           nozzle=(nz["radius"], nz["rho"], nz["u"], nz["umag"], nz["eps"], 0.0),
                           phi_seq=phis)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        O.run3(Po, mode, bc, 2)
-        steps += 2
-        el = time.perf_counter() - t0
-        if el > budget_s or steps >= 400:
-            break
+
+    def timed(mode, budget):
+        Po = np.ascontiguousarray(jet_state(n).T)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            O.run3(Po, mode, bc, 2)
+            steps += 2
+            el = time.perf_counter() - t0
+            if el > budget or steps >= 400:
+                return steps, el
+
+    ext = O.make_mode(ref=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50,
+                      tol=1e-10, gamma=GAMMA, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx)
+    steps, el = timed(ext, budget_s)
+    ref = O.make_mode(ref=True, rec="wenoz", riemann="hll", integrator="rk2", n_iter=50,
+                      tol=1e-10, gamma=GAMMA, floor=1e-7, dt=0.4 / (3.0 / dx), dx=dx, dy=dx, dz=dx)
+    steps_r, el_r = timed(ref, budget_ref_s)
     return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": cores, "kind": "port",
             "sample": f"CPU oracle (C restatement of Verona's algorithm, OpenMP) on the same jet "
-                      f"workload at {n}^3, {steps} SSP-RK3 WENOZ+HLLC steps, {el:.1f} s"}
+                      f"workload at {n}^3, {steps} SSP-RK3 WENOZ+HLLC steps, {el:.1f} s",
+            "ref_semantics_value": n ** 3 * steps_r / el_r,
+            "ref_semantics_sample": f"same, Verona's method (WENOZ+HLL, midpoint RK2, fixed dt), "
+                                    f"{steps_r} steps, {el_r:.1f} s",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
 
 
 def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None, quick=False):
@@ -367,6 +390,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "value_reference_formula": (n + 6) ** 3 * K / el,  # ghosts counted, as 3dim/algos.jl:347 prints
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3D relativistic jet {n}^3, WENOZ+HLLC+SSP-RK3, CFL 0.4, FP64",
