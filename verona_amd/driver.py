@@ -47,6 +47,47 @@ def load_dump(path):
     return data, T, grid, step, int(base[4:-4])
 
 
+def _h5py():
+    try:
+        import h5py
+    except ImportError as e:  # not in every environment; the npz dumps need nothing
+        raise ImportError("HDF5 dumps need h5py, which is not installed here; the run loop "
+                          "writes dump<i>.npz (same datasets) and dump_to_hdf5 converts them "
+                          "where h5py is available") from e
+    return h5py
+
+
+def dump_to_hdf5(npz_path, h5_path=None):
+    """Write a dump<i>.npz as the HDF5 file SaveHDF5Parallel writes
+    (src/3dim/saving.jl:57-146) and SaveHDF5Gather (src/2dim/saving.jl:2-45)
+    write: dataset ``data`` with the
+    dimensions reversed as HDF5.jl stores a Julia (nvar, Nx, Ny[, Nz]) array
+    (C shape (Nz, Ny, Nx, nvar), variable fastest: the same bytes as Julia's
+    column-major array), scalar ``T`` and ``grid`` ([dx, dy(, dz)]); ``step``
+    is kept for restarts.  Returns the path written."""
+    h5py = _h5py()
+    data, T, grid, step, _ = load_dump(npz_path)
+    h5_path = h5_path or npz_path[:-4] + ".h5"
+    with h5py.File(h5_path, "w") as f:
+        f.create_dataset("data", data=np.ascontiguousarray(data.T))
+        f.create_dataset("T", data=np.float64(T))
+        f.create_dataset("grid", data=np.asarray(grid, dtype=np.float64))
+        if step is not None:
+            f.create_dataset("step", data=np.int64(step))
+    return h5_path
+
+
+def load_hdf5_dump(h5_path):
+    """(data, T, grid) of an HDF5 dump in the reference's layout (see
+    dump_to_hdf5), ``data`` back in Julia's (nvar, Nx, Ny[, Nz]) index order."""
+    h5py = _h5py()
+    with h5py.File(h5_path, "r") as f:
+        data = np.ascontiguousarray(np.asarray(f["data"]).T)
+        T = float(np.asarray(f["T"]))
+        grid = tuple(float(g) for g in np.asarray(f["grid"]))
+    return data, T, grid
+
+
 def harm_hll(dev, ndim, comm, P_host_view, dt, grid, Tmax, eos, drops, floor, out_dir, verbose,
              n_iter, tol, fun_bound, *, rec="wenoz", riemann="hll", integrator="rk2", cfl=0.0,
              precision="fp64", ref_mode=True, max_steps=None, restart=None):
