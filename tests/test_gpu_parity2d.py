@@ -1,5 +1,4 @@
 """GPU parity of the 2D hot path (4 variables, 4x4 Newton con2prim) against the CPU oracle."""
-import numpy as np
 import pytest
 
 from helpers import from_oracle2, interior, per_var_rel_l1, progress, rel_l1, state2, to_oracle2

@@ -1,7 +1,6 @@
 """Host-side logic of the package (CPU only)."""
 import math
 
-import numpy as np
 import pytest
 
 from verona_amd import Boundary, Cart, EosTypes, host_array
