@@ -292,6 +292,23 @@ def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None
     return out
 
 
+def copy_bandwidth(n, reps=5):
+    """Device-to-device copy rate of one n^3 state slot (read + write bytes per
+    second): the practical streaming ceiling the sweeps are compared with,
+    measured on the same GPU in the same run."""
+    from verona_amd import DeviceGrid
+    with DeviceGrid(3, n, n, n, GAMMA) as dev:
+        nbytes = dev.slot_bytes()
+        dev.copy("U", "P")
+        dev.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dev.copy("U", "P")
+        dev.sync()
+        el = time.perf_counter() - t0
+    return 2.0 * nbytes * reps / el / 1e9
+
+
 def measured_traffic(kernel, n, precision):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     passes (copy-calibrated), when they were taken on this workload and form; else None."""
@@ -379,6 +396,7 @@ def main():
     per_cell = {"zsweep_c2p": 5 * 40, "xsweep": 3 * 40, "ysweep": 3 * 40}.get(dom, 4 * 40)
     ach = per_cell * cells / (mean_ms * 1e-3) / 1e9
     step_gbs = BYTES_PER_ZU["rk3"] * value / 1e9
+    copy_gbs = copy_bandwidth(n)
     out = {
         "metric": "zone-updates/s (3D WENOZ+HLLC, 1 GPU); con2prim cells/s learned vs Newton",
         "value": value,
@@ -399,6 +417,7 @@ def main():
                    "lorentz": 10.0, "bc": "nozzle z-low, outflow elsewhere"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "measured_copy_gbs": copy_gbs, "frac_of_copy": ach / copy_gbs,
                      "traffic": measured_traffic(dom, n, args.precision),
                      "traffic_source": "profiles/jet384_rk3_hbm_traffic.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)",
                      "bytes_per_cell_per_launch": per_cell, "mean_launch_ms": mean_ms,
