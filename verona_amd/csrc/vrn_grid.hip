@@ -385,9 +385,9 @@ int fail(int code, const char *fmt, ...)
             return fail(VRN_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
     } while (0)
 
-const char *kKernelNames[] = {"xsweep", "ysweep", "zsweep_c2p", "ysweep_c2p", "rescue",
+const char *kKernelNames[] = {"xsweep", "ysweep", "zsweep_c2p", "finish_2d", "rescue",
                               "ghost",  "cfl",    "fluxes",     "update",     "c2p"};
-enum { KN_X, KN_Y, KN_ZC, KN_YC, KN_RESCUE, KN_GHOST, KN_CFL, KN_FLUX, KN_UPD, KN_C2P, KN_COUNT };
+enum { KN_X, KN_Y, KN_ZC, KN_F2D, KN_RESCUE, KN_GHOST, KN_CFL, KN_FLUX, KN_UPD, KN_C2P, KN_COUNT };
 
 }  // namespace
 

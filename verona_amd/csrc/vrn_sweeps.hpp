@@ -375,6 +375,41 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         if (a.do_rate) rate_reduce(a.st, rate);
 }
 
+// ------------------------------------------------------------ 2D stage end
+// In 2D the y sweep leaves the stage's accumulated update in Uacc, like the
+// middle sweep in 3D, and this pointwise pass forms the RK combination and
+// runs the 2D Newton con2prim (2dim/structs.jl:249-330) from the stage's
+// primitives, Limit and the CFL rate -- the same arithmetic as doing it
+// inside the march.  The 4x4 Newton needs ~200 registers: inside the march
+// that left 2 waves per SIMD with one wave per 64-column segment (2048 waves
+// at 2048^2); here the cells are spread over every SIMD.
+template <int NV>
+__global__ __launch_bounds__(256) void k_finish2d(SweepArgs a)
+{
+    const Geo G = a.geo;
+    const int nx = G.NX - 6;
+    const long n = (long)nx * (G.NY - 6);
+    const long ps = G.ps;
+    double rate = 0.0;
+    // block-uniform loop bound: every lane of a wave takes the same trips,
+    // so the wave reduction of the rate below sees all 64 lanes
+    for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
+        const long t = base + threadIdx.x;
+        const bool valid = t < n;
+        const long tt = valid ? t : 0;
+        const long cu = gidx(G, 3 + (int)(tt % nx), 3 + (int)(tt / nx), 0);
+        double acc[NV], u0[NV], guess[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            acc[v] = valid ? a.Uacc[v * ps + cu] : 0.0;
+            u0[v] = (valid && a.a != 0.0) ? a.U0[v * ps + cu] : 0.0;
+            guess[v] = valid ? a.P[v * ps + cu] : 1.0;
+        }
+        finish_cell<NV>(a, cu, acc, u0, guess, valid, rate);
+    }
+    if (a.do_rate) rate_reduce(a.st, rate);
+}
+
 // ------------------------------------------------------------ rescue (3D)
 // Cells whose 1DW Newton did not converge: rest of the cascade from the
 // stage's guess, then Limit.  The list holds linear device indices.
@@ -445,8 +480,13 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
         SweepArgs ay = a;
         ay.dx_dim = dxs[1];
         int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
+        timer(tctx, 1, 0);
+        hipLaunchKernelGGL((k_march<4, 1, R, REC, RIEM, false>), dim3(nxb, 1, nseg), dim3(64), 0, s, ay);
+        timer(tctx, 1, 1);
+        const long n = (long)(G.NX - 6) * (G.NY - 6);
+        const long blocks = (n + 255) / 256;
         timer(tctx, 3, 0);
-        hipLaunchKernelGGL((k_march<4, 1, R, REC, RIEM, true>), dim3(nxb, 1, nseg), dim3(64), 0, s, ay);
+        hipLaunchKernelGGL(k_finish2d<4>, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(256), 0, s, a);
         timer(tctx, 3, 1);
     }
 }
