@@ -818,29 +818,50 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
 {
     for (int it = 0; it < n_iter; it++) {
         iters++;
-        double gam = dsqrt(P[2] * P[2] + P[3] * P[3] + 1.0);
+        double gam = vrn_sqrt1(P[2] * P[2] + P[3] * P[3] + 1.0);
         double w = g * P[1] + P[0];
         double f[4], A[16], o[4], rn[4];
+#if defined(VRN_FAST)
+        // fast form: one reciprocal of gamma, of each row maximum and of each
+        // pivot instead of the 19 divisions per iteration
+        const double ig = recip(gam);
+        double rp[4];
+#endif
         f[0] = gam * P[0] - U[0];
         f[1] = (g - 1.0) * P[1] - gam * gam * w - U[1];
         f[2] = P[2] * gam * w - U[2];
         f[3] = P[3] * gam * w - U[3];
         A[0] = gam;
         A[4] = 0.0;
+#if defined(VRN_FAST)
+        A[8] = P[0] * P[2] * ig;
+        A[12] = P[0] * P[3] * ig;
+#else
         A[8] = P[0] * P[2] / gam;
         A[12] = P[0] * P[3] / gam;
+#endif
         A[1] = -(gam * gam);
         A[5] = (g - 1.0) - gam * gam * g;
         A[9] = -2.0 * P[2] * w;
         A[13] = -2.0 * P[3] * w;
         A[2] = gam * P[2];
         A[6] = gam * P[2] * g;
+#if defined(VRN_FAST)
+        A[10] = (2.0 * (P[2] * P[2]) + P[3] * P[3] + 1.0) * ig * w;
+        A[14] = P[2] * P[3] * ig * w;
+#else
         A[10] = (2.0 * (P[2] * P[2]) + P[3] * P[3] + 1.0) / gam * w;
         A[14] = P[2] * P[3] / gam * w;
+#endif
         A[3] = gam * P[3];
         A[7] = gam * P[3] * g;
+#if defined(VRN_FAST)
+        A[11] = P[2] * P[3] * ig * w;
+        A[15] = (2.0 * (P[3] * P[3]) + P[2] * P[2] + 1.0) * ig * w;
+#else
         A[11] = P[2] * P[3] / gam * w;
         A[15] = (2.0 * (P[3] * P[3]) + P[2] * P[2] + 1.0) / gam * w;
+#endif
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             double am = 0.0;
@@ -850,7 +871,7 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
                 if (t > am) am = t;
             }
             if (am == 0.0) return false;
-            rn[i] = 1.0 / am;
+            rn[i] = recip(am);
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -882,9 +903,16 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
                 rn[k] = s ? b : a;
                 rn[i] = s ? a : b;
             }
+#if defined(VRN_FAST)
+            rp[k] = recip(A[k * 4 + k]);
+#endif
 #pragma unroll
             for (int i = k + 1; i < 4; i++) {
+#if defined(VRN_FAST)
+                A[k * 4 + i] *= rp[k];
+#else
                 A[k * 4 + i] /= A[k * 4 + k];
+#endif
 #pragma unroll
                 for (int j = k + 1; j < 4; j++)
                     A[j * 4 + i] = __builtin_fma(-A[k * 4 + i], A[j * 4 + k], A[j * 4 + i]);
@@ -900,9 +928,17 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
         for (int i = 3; i >= 0; i--) {
 #pragma unroll
             for (int j = i + 1; j < 4; j++) o[i] = __builtin_fma(-A[j * 4 + i], o[j], o[i]);
+#if defined(VRN_FAST)
+            o[i] *= rp[i];
+#else
             o[i] /= A[i * 4 + i];
+#endif
         }
+#if defined(VRN_FAST)
+        if (o[0] * o[0] + o[1] * o[1] + o[2] * o[2] + o[3] * o[3] < tol * tol) break;
+#else
         if (dsqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2] + o[3] * o[3]) < tol) break;
+#endif
 #pragma unroll
         for (int v = 0; v < 4; v++) P[v] = P[v] - o[v];
     }
