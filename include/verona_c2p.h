@@ -102,7 +102,8 @@ int vrn_mlp_infer(vrn_c2p *h, int64_t n, const float *X, float *Y);
 /* Loss (mean over samples of sum_o w_o (y_o - t_o)^2) and its gradient
  * with respect to params, by the training kernels.  qat = 1 evaluates the
  * network the way the inference kernel does (BF16 rounding of the inputs,
- * W1, W2, W3 and the first hidden layer; straight-through gradients). */
+ * W1 with b1, W2, W3 and both hidden layers; FP32 b2, b3 and accumulation;
+ * straight-through gradients). */
 int vrn_mlp_grad(vrn_c2p *h, int64_t n, const float *X, const float *T, const float *params,
                  int qat, float *loss, float *grad);
 
