@@ -67,7 +67,10 @@ typedef struct vrn_ctx vrn_ctx;
 #define VRN_BC_PERIODIC 1 /* single-rank periodic halo exchange */
 #define VRN_BC_OUTFLOW 2  /* zero-gradient copy (examples/Jet3D TurnOff) */
 #define VRN_BC_NOZZLE 3   /* z-low only: outflow plus jet nozzle inflow */
-/* buffer slots of a context (the arrays HARM_HLL allocates, src/3dim/algos.jl:224-228) */
+/* buffer slots of a context (the arrays HARM_HLL allocates, src/3dim/algos.jl:224-228);
+ * the flux slots are allocated (zeroed) by the first call that uses them --
+ * vrn_fluxes, vrn_update, or an upload/download/copy naming them -- since
+ * vrn_step never does */
 #define VRN_SLOT_P 0
 #define VRN_SLOT_U 1
 #define VRN_SLOT_UHALF 2
@@ -149,7 +152,8 @@ typedef struct {
     int precision;  /* VRN_PREC_* */
     int n_iter;     /* con2prim iteration limit (kwargs[1]) */
     double tol;     /* con2prim tolerance (kwargs[2]) */
-    double floor_;  /* density / internal energy floor */
+    double floor_;  /* density / internal energy floor (a number, not -0:
+                       vrn_step returns VRN_E_ARG otherwise) */
     double dt;      /* fixed dt; ignored when cfl > 0 */
     double cfl;     /* > 0: dt = cfl / max_cells sum_d max|lambda_d|/dx_d */
     double dx, dy, dz;
