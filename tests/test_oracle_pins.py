@@ -374,3 +374,36 @@ def test_time_accumulation_step_count(oracle):
         tt += 0.4 * dx
         k += 1
     assert steps == k and t == tt
+
+
+def test_near_vacuum_front_is_ill_conditioned_2d(oracle):
+    """Why the fast FP64 form is not held to 1e-12 on a near-vacuum front: in
+    the exact algorithm itself, a 1e-16 relative perturbation of the input
+    moves the 2D velocities by far more than 1e-12 after one step, while the
+    densities stay at rounding level and the same state without the stripe
+    stays well conditioned."""
+    from helpers import from_oracle2, interior, state2, to_oracle2
+    nx, ny = 37, 53
+    dx, dy = 1 / nx, 1 / ny
+    m = oracle.make_mode(ref=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50, tol=1e-10,
+                         gamma=G3, floor=1e-7, cfl=0.4, dx=dx, dy=dy)
+
+    def spread(stripe):
+        P = state2(nx, ny, seed=9)
+        if stripe:
+            x = np.arange(P.shape[1])[:, None]
+            low = (x > 10) & (x < 16)
+            P[0] = np.where(low, 4e-7, P[0])
+            P[1] = np.where(low, 3e-7, P[1])
+        out = []
+        for eps in (0.0, 1e-16):
+            Q = np.asfortranarray(P * (1 + eps * np.random.default_rng(2).standard_normal(P.shape)))
+            Po = to_oracle2(Q)
+            oracle.run2(Po, m, [2, 2, 1, 1], 1)
+            out.append(interior(from_oracle2(Po)))
+        a, b = out
+        return [np.abs(b[v] - a[v]).sum() / np.abs(a[v]).sum() for v in range(4)]
+
+    s = spread(True)
+    assert s[0] < 1e-12 and max(s[2], s[3]) > 1e-8
+    assert max(spread(False)) < 1e-14

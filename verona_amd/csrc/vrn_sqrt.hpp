@@ -41,11 +41,19 @@ __host__ __device__ __forceinline__ double vrn_sqrt(double x)
 #endif
 }
 
-// sqrt(x) for finite x >= 1 (Lorentz factors sqrt(1 + |u|^2)) or NaN.
+// sqrt(x) for x >= 1, +inf or NaN (Lorentz factors sqrt(1 + |u|^2)).  +inf
+// is reached when |u|^2 overflows (|u| > 1.3e154, e.g. a diverging Newton
+// iterate); the core gives NaN there (inf * rsq(inf) = inf * 0).  z = x 2^-1022
+// is +inf for x = +inf, NaN for NaN x, and for finite x >= 0 it is at most
+// sqrt(x) (x 2^-1022 <= sqrt(x) <=> x <= 2^2044), hence at most the correctly
+// rounded sqrt(x); so IEEE maxNum(core, z) restores sqrt(+inf) = +inf and
+// returns the core's value everywhere else (two instructions and no compare
+// mask: a class-test select made the 128-register x sweep spill).  Negative
+// arguments are outside the domain (they would return z, not NaN).
 __host__ __device__ __forceinline__ double vrn_sqrt1(double x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return vrn_sqrt_core(x);
+    return __builtin_fmax(vrn_sqrt_core(x), x * 0x1p-1022);
 #else
     return __builtin_sqrt(x);
 #endif
