@@ -121,7 +121,7 @@ def run_kh(n=2048, steps=40, warmup=3, precision="fp64fast"):
             "ms_per_step": el / steps * 1e3, "n_unrecovered": nbad}
 
 
-def run_tubes(ntubes=4096, steps=500):
+def run_tubes(ntubes=4096, steps=500, precision="fp64", keep=False):
     """Batched uneven 1D shock tubes: 4096 tubes, lengths U{256..8192} (SplitMix64 seed
     42), PPM + HLLC, RK2, dt = 0.4 dx per tube, one persistent hipGraph launch with the
     autotuned launch shape.  value = cells x steps / s of the timed launch."""
@@ -130,21 +130,44 @@ def run_tubes(ntubes=4096, steps=500):
     with Tubes(lengths) as tb:
         tb.upload(rho, u, ux, dt, dx)
         tb.set_method("ppm", "hllc", 50, 1e-10)
+        tb.set_precision(precision)
         block, grid = tb.autotune(probe_steps=2)
         tb.upload(rho, u, ux, dt, dx)
         tb.run(2, block, grid)  # graph capture + warm-up
         tb.upload(rho, u, ux, dt, dx)
         ms = tb.run(steps, block, grid)
-        r, _, _ = tb.download()
+        r, uu, vx = tb.download()
     cells = int(lengths.sum())
     off = np.concatenate([[0], np.cumsum(lengths)])
     # Verona1D's con2prim has no floors: tubes whose random states it cannot
     # recover end with NaN markers (the oracle marks the same cells)
     bad = int(sum(not np.isfinite(r[off[t]:off[t + 1]]).all() for t in range(ntubes)))
-    return {"what": f"{ntubes} uneven 1D tubes ({cells} cells), PPM+HLLC, RK2, {steps} steps, "
-                    "one persistent hipGraph launch", "value": cells * steps / (ms * 1e-3),
-            "unit": "cell-steps/s", "ms": ms, "launch_shape": [block, grid],
-            "tubes_with_nan_markers": bad}
+    out = {"what": f"{ntubes} uneven 1D tubes ({cells} cells), PPM+HLLC, RK2, {steps} steps, "
+                   "one persistent hipGraph launch", "form": precision, "value": cells * steps / (ms * 1e-3),
+           "unit": "cell-steps/s", "ms": ms, "launch_shape": [block, grid],
+           "tubes_with_nan_markers": bad}
+    if keep:
+        out["_state"] = (off, np.stack([r, uu, vx]))
+    return out
+
+
+def run_tubes_both(ntubes=4096, steps=500):
+    """The tubes leg in the exact FP64 form (the line's value) and in the fast
+    FP64 form, with the fast form's relative L1 difference from the exact one
+    over the tubes both leave finite."""
+    ex = run_tubes(ntubes, steps, "fp64", keep=True)
+    fa = run_tubes(ntubes, steps, "fp64fast", keep=True)
+    off, A = ex.pop("_state")
+    _, B = fa.pop("_state")
+    ok = np.ones(len(off) - 1, bool)
+    for t in range(len(off) - 1):
+        ok[t] = np.isfinite(A[:, off[t]:off[t + 1]]).all() and np.isfinite(B[:, off[t]:off[t + 1]]).all()
+    m = np.repeat(ok, np.diff(off))
+    fa["rel_l1_vs_exact"] = [float(np.abs(B[v][m] - A[v][m]).sum() / np.abs(A[v][m]).sum()) for v in range(3)]
+    fa["tubes_compared"] = int(ok.sum())
+    ex["fast_form"] = {k: fa[k] for k in ("value", "unit", "ms", "launch_shape", "tubes_with_nan_markers",
+                                          "rel_l1_vs_exact", "tubes_compared")}
+    return ex
 
 
 def fp32_error(n, steps):
@@ -391,7 +414,7 @@ def main():
         print(json.dumps({"con2prim": run_c2p(log=log, quick=args.c2p_quick)}), flush=True)
         return
     if args.tubes_only:
-        print(json.dumps({"batched_tubes": run_tubes(steps=args.tubes_steps)}), flush=True)
+        print(json.dumps({"batched_tubes": run_tubes_both(steps=args.tubes_steps)}), flush=True)
         return
     cells = n ** 3
     el, rep, nbad, nresc = run_gpu(n, K, W, ref=False, precision=args.precision)
@@ -460,7 +483,7 @@ def main():
         log("kelvin-helmholtz")
         out["kelvin_helmholtz"] = run_kh()
         log("batched tubes")
-        out["batched_tubes"] = run_tubes()
+        out["batched_tubes"] = run_tubes_both()
     if not args.no_c2p:
         out["con2prim"] = run_c2p(log=log)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline()

@@ -205,6 +205,12 @@ int vrn_tubes_upload(vrn_tubes *tb, const double *rho, const double *u, const do
 int vrn_tubes_download(vrn_tubes *tb, double *rho, double *u, double *ux);
 /* rec (VRN_REC_*), riemann (VRN_RIEMANN_*), n_iter and tol (kwargs of HARM_HLL). */
 int vrn_tubes_set_method(vrn_tubes *tb, int rec, int riemann, int n_iter, double tol);
+/* Arithmetic of the tube kernels: VRN_PREC_FP64 (default; Verona1D's FP64
+ * operation order, bit-reproducible) or VRN_PREC_FP64_FAST (FMA contraction,
+ * reciprocal-based division, squared-norm Newton stop).  No counterpart in
+ * the reference, whose 1D solver runs one FP64 form; VRN_PREC_FP32 is
+ * refused with VRN_E_ARG. */
+int vrn_tubes_set_precision(vrn_tubes *tb, int precision);
 /* Advance all tubes by nsteps midpoint-RK2 steps with `grid` persistent
  * workgroups of `block` threads (multiple of 64); use_graph replays a
  * captured hipGraph.  *ms receives the device time.  Blocks. */

@@ -60,6 +60,52 @@ def test_batch_matches_oracle_per_tube(oracle, rec, riemann):
         assert np.abs(G[v][f] - O[v][f]).sum() <= 1e-12 * np.abs(O[v][f]).sum()
 
 
+@pytest.mark.parametrize("rec,riemann", [("ppm", "hllc"), ("wenoz", "hll")])
+def test_fast_form_matches_oracle_per_tube(oracle, rec, riemann):
+    """The fast FP64 form (FMA contraction, reciprocal division, squared-norm
+    Newton stop) against the oracle to 1e-12 relative L1 on the tubes both
+    leave finite.  Then, on the same object, switching back to the exact form
+    and replaying a captured graph of the same shape must rebuild the graph:
+    the result is bit-identical to the oracle again."""
+    from verona_amd.tubes import Tubes, shock_tube_batch
+    progress(f"test_fast_form_matches_oracle_per_tube {rec} {riemann}")
+    lengths, rho, u, ux, dt, dx = shock_tube_batch(48, lmin=40, lmax=700, seed=7)
+    nsteps = 40
+    O = _oracle_tubes(oracle, lengths, rho, u, ux, dt, dx, 5.0 / 3.0, rec, riemann, nsteps)
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    with Tubes(lengths) as tb:
+        tb.upload(rho, u, ux, dt, dx)
+        tb.set_method(rec, riemann, 50, 1e-10)
+        tb.set_precision("fp64fast")
+        tb.run(nsteps, block=128, grid=16)
+        G = np.stack(tb.download())
+        tb.upload(rho, u, ux, dt, dx)
+        tb.set_precision("fp64")
+        tb.run(nsteps, block=128, grid=16)
+        E = np.stack(tb.download())
+    fin = lambda X: np.array([np.isfinite(X[:, off[t]:off[t + 1]]).all() for t in range(len(lengths))])
+    fo, fg = fin(O), fin(G)
+    m = np.repeat(fo & fg, lengths)
+    errs = [np.abs(G[v][m] - O[v][m]).sum() / np.abs(O[v][m]).sum() for v in range(3)]
+    progress(f"  tubes finite: oracle {int(fo.sum())}, fast {int(fg.sum())}, both {int((fo & fg).sum())}; "
+             f"rel L1 {['%.2e' % e for e in errs]}")
+    assert (fo & fg).sum() >= 0.9 * fo.sum()
+    assert max(errs) <= 1e-12
+    fe = np.isfinite(O)
+    assert np.array_equal(np.isfinite(E), fe) and np.array_equal(E[fe], O[fe])
+
+
+def test_tubes_refuse_fp32():
+    from verona_amd._lib import VeronaHIPError
+    from verona_amd.tubes import Tubes, _check
+    progress("test_tubes_refuse_fp32")
+    with Tubes(np.full(4, 16)) as tb:
+        with pytest.raises(ValueError):
+            tb.set_precision("fp32")
+        with pytest.raises(VeronaHIPError, match="VRN_PREC_FP64"):
+            _check(tb.lib.vrn_tubes_set_precision(tb.h, 1), "vrn_tubes_set_precision")
+
+
 def test_graph_replay_and_autotune_are_transparent(oracle):
     """Graph replays (several launches in a row) and an autotune pass (which
     checkpoints and restores the state) give the same answer as one eager run."""

@@ -1,4 +1,4 @@
-// vrn_types.hpp -- device layout and argument blocks shared by the grid kernels.
+// vrn_types.hpp -- device layout and argument blocks shared by the grid and tube kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -55,6 +55,25 @@ struct SweepArgs {
     int seg_len;        // y / z sweeps: interior cells per marching segment
     int rows_per_wave;  // x sweep: rows streamed by one wave (xseg == 1)
     int xseg;           // x sweep: segments per row (> 1 only with rows_per_wave == 1)
+};
+
+// Batched 1D tubes (one persistent launch, one tube per workgroup at a time).
+struct TubeArgs {
+    double *P[3];       // primitives P^n (in/out)
+    double *Ph[3];      // stage-1 primitives
+    double *U[3];       // conserved U^n (in/out)
+    double *Uh[3];      // Uhalf
+    const double *P0[3];  // initial primitives (stale face states)
+    const long *off;    // cell offset of each tube, [ntubes + 1]
+    const int *order;   // tube processing order (longest first)
+    const double *dt;   // per-tube dt
+    const double *dx;   // per-tube dx
+    unsigned int *queue;  // work counter
+    int ntubes;
+    int nsteps;
+    int n_iter;
+    double tol;
+    double g;
 };
 
 }  // namespace vrn

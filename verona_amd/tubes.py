@@ -32,6 +32,7 @@ TUBE_SIGNATURES = {
     "vrn_tubes_upload": (_I, [_P, _lib._DP, _lib._DP, _lib._DP, _lib._DP, _lib._DP]),
     "vrn_tubes_download": (_I, [_P, _lib._DP, _lib._DP, _lib._DP]),
     "vrn_tubes_set_method": (_I, [_P, _I, _I, _I, _D]),
+    "vrn_tubes_set_precision": (_I, [_P, _I]),
     "vrn_tubes_run": (_I, [_P, _I, _I, _I, _I, _lib._DP]),
     "vrn_tubes_checkpoint": (_I, [_P, _I]),
     "vrn_tubes_info": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(_I)]),
@@ -64,6 +65,7 @@ class Tubes:
         self.h = h
         self.gamma = float(gamma)
         self.method = None
+        self.precision = "fp64"
         n = C.c_int64()
         cu = _I()
         _check(self.lib.vrn_tubes_info(self.h, C.byref(n), C.byref(cu)), "vrn_tubes_info")
@@ -107,6 +109,15 @@ class Tubes:
                "vrn_tubes_set_method")
         self.method = (rec, riemann, int(n_iter), float(tol))
 
+    def set_precision(self, precision="fp64"):
+        """"fp64" (default): Verona1D's FP64 operation order, bit-reproducible;
+        "fp64fast": FMA contraction and reciprocal-based division."""
+        codes = {"fp64": 0, "fp64fast": 2}
+        if precision not in codes:
+            raise ValueError(f"precision must be one of {sorted(codes)}, got {precision!r}")
+        _check(self.lib.vrn_tubes_set_precision(self.h, codes[precision]), "vrn_tubes_set_precision")
+        self.precision = precision
+
     def run(self, nsteps, block=128, grid=None, use_graph=True):
         if grid is None:
             grid = 2 * self.cu_count
@@ -119,7 +130,7 @@ class Tubes:
     def signature(self):
         m = hashlib.sha1()
         m.update(self.lengths.tobytes())
-        m.update(repr((self.gamma, self.method, self.cu_count)).encode())
+        m.update(repr((self.gamma, self.method, self.precision, self.cu_count)).encode())
         return m.hexdigest()
 
     def autotune(self, probe_steps=2, cache_file=None, verbose=False):
