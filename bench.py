@@ -388,6 +388,7 @@ def main():
     ap.add_argument("--c2p-quick", action="store_true",
                     help="with --c2p-only: the short profiling sequence (800 training steps, 2 solves per guess)")
     ap.add_argument("--tubes-only", action="store_true", help="only the batched 1D tubes leg")
+    ap.add_argument("--kh-only", action="store_true", help="only the 2D Kelvin-Helmholtz 2048^2 leg")
     ap.add_argument("--tubes-steps", type=int, default=500, help="steps of the batched tubes leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of one 384^3 slot (known bytes) for counter calibration")
@@ -415,6 +416,9 @@ def main():
         return
     if args.tubes_only:
         print(json.dumps({"batched_tubes": run_tubes_both(steps=args.tubes_steps)}), flush=True)
+        return
+    if args.kh_only:
+        print(json.dumps({"kelvin_helmholtz": run_kh()}), flush=True)
         return
     cells = n ** 3
     el, rep, nbad, nresc = run_gpu(n, K, W, ref=False, precision=args.precision)
