@@ -16,6 +16,10 @@
 // synchronisation is needed.  Within a tube the workgroup streams over the
 // cells in chunks of blockDim cells, one cell per lane, exchanging face
 // states and fluxes through LDS with a carry across chunks.
+//
+// This file holds the C ABI and the exact-form kernels; the kernels
+// themselves are in vrn_tubes_kern.hpp, compiled again as the fast FP64 form
+// in vrn_tubes_fast.hip (vrn_tubes_set_precision selects one).
 #include "../../include/verona_hip.h"
 #include "vrn_tubes_kern.hpp"
 
@@ -29,8 +33,8 @@
 #include <string>
 #include <vector>
 
+// fast FP64 form of the tube kernels (vrn_tubes_fast.hip)
 void vrn_fast_launch_tubes(hipStream_t s, const vrn::TubeArgs &a, int rec, int riem, int block, int grid);
-
 
 using namespace vrn;
 
