@@ -48,6 +48,25 @@ template <class T> __device__ __forceinline__ T jsign(T x)
 }
 template <class T> __device__ __forceinline__ T sq(T x) { return x * x; }
 
+// PPM's limiter tests, written with plain compares.  same_sign3/4 equal
+// Julia's sign(a) == sign(b) == ...: true when all arguments are < 0, all
+// > 0 or all == 0 (+-0), false as soon as one is NaN.  nn_min / nn_max are
+// Julia's min / max for arguments that are not NaN and not -0 (the callers
+// pass absolute values, scaled or not), where a compare-select is exact.
+template <class T> __device__ __forceinline__ bool same_sign3(T a, T b, T c)
+{
+    return ((a < T(0)) & (b < T(0)) & (c < T(0))) | ((a > T(0)) & (b > T(0)) & (c > T(0))) |
+           ((a == T(0)) & (b == T(0)) & (c == T(0)));
+}
+template <class T> __device__ __forceinline__ bool same_sign4(T a, T b, T c, T d)
+{
+    return ((a < T(0)) & (b < T(0)) & (c < T(0)) & (d < T(0))) |
+           ((a > T(0)) & (b > T(0)) & (c > T(0)) & (d > T(0))) |
+           ((a == T(0)) & (b == T(0)) & (c == T(0)) & (d == T(0)));
+}
+template <class T> __device__ __forceinline__ T nn_min(T x, T y) { return y < x ? y : x; }
+template <class T> __device__ __forceinline__ T nn_max(T x, T y) { return y > x ? y : x; }
+
 __host__ __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
 __host__ __device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
 __host__ __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
@@ -210,6 +229,22 @@ template <class T> __device__ __forceinline__ void minmod(T qm1, T q, T qp1, T &
     hi = q + dU;
 }
 
+// x / 6.  Fast FP64 form: x (1/6) plus one remainder correction (Markstein:
+// with 1/6 rounded to nearest and a first quotient within an ulp, the
+// corrected quotient is the correctly rounded x / 6 away from the subnormal
+// range); otherwise the IEEE division.
+template <class T> __device__ __forceinline__ T div6(T x)
+{
+#if defined(VRN_FAST)
+    if constexpr (std::is_same<T, double>::value) {
+        constexpr double c = 1.0 / 6.0;
+        const double q = x * c;
+        return __builtin_fma(__builtin_fma(-q, 6.0, x), c, q);
+    }
+#endif
+    return x / T(6);
+}
+
 // fluxlimiter.jl:74-180  PPM (AthenaPK form)
 template <class T>
 __device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T &lo, T &hi)
@@ -220,8 +255,8 @@ __device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T
     T dd_im1 = T(0.5) * qa + T(0.5) * (q_im1 - q_im2);
     T dd = T(0.5) * qb + T(0.5) * qa;
     T dd_ip1 = T(0.5) * (q_ip2 - q_ip1) + T(0.5) * qb;
-    T dph = T(0.5) * (q_im1 + q_i) + (dd_im1 - dd) / T(6);
-    T dph_ip1 = T(0.5) * (q_i + q_ip1) + (dd - dd_ip1) / T(6);
+    T dph = T(0.5) * (q_im1 + q_i) + div6(dd_im1 - dd);
+    T dph_ip1 = T(0.5) * (q_i + q_ip1) + div6(dd - dd_ip1);
     T d2qc_im1 = q_im2 + q_i - T(2) * q_im1;
     T d2qc = q_im1 + q_ip1 - T(2) * q_i;
     T d2qc_ip1 = q_i + q_ip2 - T(2) * q_ip1;
@@ -232,9 +267,11 @@ __device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T
     qb = d2qc_im1;
     T qc = d2qc;
     T qd = T(0);
-    if (jsign(qa) == jsign(qb) && jsign(qa) == jsign(qc))
-        qd = jsign(qa) * jmin(C2 * dabs(qb), jmin(C2 * dabs(qc), dabs(qa)));
-    T dph_tmp = T(0.5) * (q_im1 + q_i) - qd / T(6);
+    // sign(qa) * min(...) with equal signs: the min is +0 exactly when qa is
+    // +-0, so the product is copysign(min, qa)
+    if (same_sign3(qa, qb, qc))
+        qd = __builtin_copysign(nn_min(C2 * dabs(qb), nn_min(C2 * dabs(qc), dabs(qa))), qa);
+    T dph_tmp = T(0.5) * (q_im1 + q_i) - div6(qd);
     if (qa_tmp * qb_tmp < T(0)) dph = dph_tmp;
 
     qa_tmp = dph_ip1 - q_i;
@@ -243,9 +280,9 @@ __device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T
     qb = d2qc;
     qc = d2qc_ip1;
     qd = T(0);
-    if (jsign(qa) == jsign(qb) && jsign(qa) == jsign(qc))
-        qd = jsign(qa) * jmin(C2 * dabs(qb), jmin(C2 * dabs(qc), dabs(qa)));
-    T dphip1_tmp = T(0.5) * (q_i + q_ip1) - qd / T(6);
+    if (same_sign3(qa, qb, qc))
+        qd = __builtin_copysign(nn_min(C2 * dabs(qb), nn_min(C2 * dabs(qc), dabs(qa))), qa);
+    T dphip1_tmp = T(0.5) * (q_i + q_ip1) - div6(qd);
     if (qa_tmp * qb_tmp < T(0)) dph_ip1 = dphip1_tmp;
 
     T d2qf = T(6) * (dph + dph_ip1 - T(2) * q_i);
@@ -259,12 +296,14 @@ __device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T
     qc = d2qc_ip1;
     qd = d2qf;
     T qe = T(0);
-    if (jsign(qa) == jsign(qb) && jsign(qa) == jsign(qc) && jsign(qa) == jsign(qd))
-        qe = jsign(qd) * jmin(jmin(C2 * dabs(qa), C2 * dabs(qb)), jmin(C2 * dabs(qc), dabs(qd)));
-    qa = jmax(dabs(q_im1), dabs(q_im2));
-    qb = jmax(jmax(dabs(q_i), dabs(q_ip1)), dabs(q_ip2));
+    if (same_sign4(qa, qb, qc, qd))
+        qe = __builtin_copysign(nn_min(nn_min(C2 * dabs(qa), C2 * dabs(qb)), nn_min(C2 * dabs(qc), dabs(qd))), qd);
+    // a NaN among the five inputs makes qd = d2qf NaN, so the test below is
+    // false whatever the maximum: Julia's NaN-propagating max is not needed
+    qa = nn_max(dabs(q_im1), dabs(q_im2));
+    qb = nn_max(nn_max(dabs(q_i), dabs(q_ip1)), dabs(q_ip2));
     T rho = T(0);
-    if (dabs(qd) > T(1.0e-12) * jmax(qa, qb)) rho = qe / qd;
+    if (dabs(qd) > T(1.0e-12) * nn_max(qa, qb)) rho = qe / qd;
     T tmp_m = q_i - rho * dqf_minus;
     T tmp_p = q_i + rho * dqf_plus;
     T tmp2_m = q_i - T(2) * dqf_plus;
