@@ -6,6 +6,7 @@
 #include "vrn_types.hpp"
 #include "vrn_physics.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace VRN_NS {
@@ -396,17 +397,18 @@ template <int NV>
 __global__ __launch_bounds__(256) void k_finish2d(SweepArgs a)
 {
     const Geo G = a.geo;
-    const int nx = G.NX - 6;
-    const long n = (long)nx * (G.NY - 6);
+    const int nx = G.NX - 6, ny = G.NY - 6;
     const long ps = G.ps;
     double rate = 0.0;
-    // block-uniform loop bound: every lane of a wave takes the same trips,
-    // so the wave reduction of the rate below sees all 64 lanes
-    for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
-        const long t = base + threadIdx.x;
-        const bool valid = t < n;
-        const long tt = valid ? t : 0;
-        const long cu = gidx(G, 3 + (int)(tt % nx), 3 + (int)(tt / nx), 0);
+    // x from the block column and lane, rows strided over blockIdx.y: no
+    // integer division per cell.  The row loop bound is block-uniform, so
+    // every lane of a wave takes the same trips and the wave reduction of
+    // the rate below sees all 64 lanes.
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool xin = x < nx;
+    for (int y = blockIdx.y; y < ny; y += gridDim.y) {
+        const bool valid = xin;
+        const long cu = gidx(G, 3 + (xin ? x : 0), 3 + y, 0);
         double acc[NV], u0[NV], guess[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -492,10 +494,11 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
         timer(tctx, 1, 0);
         hipLaunchKernelGGL((k_march<4, 1, R, REC, RIEM, false>), dim3(nxb, 1, nseg), dim3(64), 0, s, ay);
         timer(tctx, 1, 1);
-        const long n = (long)(G.NX - 6) * (G.NY - 6);
-        const long blocks = (n + 255) / 256;
+        // about 2048 blocks of 256 lanes: x columns of 256 cells by row groups
+        const int bx = (G.NX - 6 + 255) / 256;
+        const int by = std::max(1, std::min(G.NY - 6, 2048 / bx));
         timer(tctx, 3, 0);
-        hipLaunchKernelGGL(k_finish2d<4>, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_finish2d<4>, dim3((unsigned)bx, (unsigned)by), dim3(256), 0, s, a);
         timer(tctx, 3, 1);
     }
 }
