@@ -1,5 +1,5 @@
 """bench.py's command-line contract on the CPU: more than one GPU is refused
-with a message (this build shards nothing), and the CPU-baseline leg returns
+with a message (the library runs on one GPU), and the CPU-baseline leg returns
 the fields the JSON line carries."""
 import json
 import os
