@@ -907,7 +907,10 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 double t = dabs(A[j * 4 + i]);
-                if (t > am) am = t;
+                // `if (t > am) am = t` of the reference: am starts at +0 and
+                // t = |A| is >= +0 or a quiet NaN, which both forms skip, so
+                // IEEE maxNum gives the same value in one instruction
+                am = __builtin_fmax(am, t);
             }
             if (am == 0.0) return false;
             rn[i] = recip(am);
