@@ -928,11 +928,16 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
                 }
             }
             // Row swap written as selects so the arrays stay in registers.
+            // The right-hand side is eliminated along with the matrix (the
+            // forward substitution's fma(-L_ij, x_j, x_i) for j = 0..i-1, in
+            // that order, as each step finishes), so the multipliers of the
+            // columns before k are dead and only columns k..3 move; rn[k] is
+            // not read after this step.
 #pragma unroll
             for (int i = k + 1; i < 4; i++) {
                 bool s = (piv == i);
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
+                for (int j = k; j < 4; j++) {
                     double a = A[j * 4 + k], b = A[j * 4 + i];
                     A[j * 4 + k] = s ? b : a;
                     A[j * 4 + i] = s ? a : b;
@@ -940,10 +945,7 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
                 double a = f[k], b = f[i];
                 f[k] = s ? b : a;
                 f[i] = s ? a : b;
-                a = rn[k];
-                b = rn[i];
-                rn[k] = s ? b : a;
-                rn[i] = s ? a : b;
+                rn[i] = s ? rn[k] : rn[i];
             }
 #if defined(VRN_FAST)
             rp[k] = recip(A[k * 4 + k]);
@@ -951,23 +953,18 @@ __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int
 #pragma unroll
             for (int i = k + 1; i < 4; i++) {
 #if defined(VRN_FAST)
-                A[k * 4 + i] *= rp[k];
+                const double m = A[k * 4 + i] * rp[k];
 #else
-                A[k * 4 + i] /= A[k * 4 + k];
+                const double m = A[k * 4 + i] / A[k * 4 + k];
 #endif
 #pragma unroll
-                for (int j = k + 1; j < 4; j++)
-                    A[j * 4 + i] = __builtin_fma(-A[k * 4 + i], A[j * 4 + k], A[j * 4 + i]);
+                for (int j = k + 1; j < 4; j++) A[j * 4 + i] = __builtin_fma(-m, A[j * 4 + k], A[j * 4 + i]);
+                f[i] = __builtin_fma(-m, f[k], f[i]);
             }
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            o[i] = f[i];
-#pragma unroll
-            for (int j = 0; j < i; j++) o[i] = __builtin_fma(-A[j * 4 + i], o[j], o[i]);
-        }
-#pragma unroll
         for (int i = 3; i >= 0; i--) {
+            o[i] = f[i];
 #pragma unroll
             for (int j = i + 1; j < 4; j++) o[i] = __builtin_fma(-A[j * 4 + i], o[j], o[i]);
 #if defined(VRN_FAST)
