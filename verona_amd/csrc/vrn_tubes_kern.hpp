@@ -173,6 +173,8 @@ __device__ void tube_stage(const vrn::TubeArgs &a, int tb, int s, double *lds)
         __syncthreads();
         if (face) {
             const int j = c - 1;  // cell updated / recovered by this lane
+            // the stage's outputs are written once per step: non-temporal
+            // stores (same-box A/B: +0.9 % exact form, +1-2 % fast form)
             double Uo[3];
             const bool upd = j >= 1 && j < N - 2;
 #pragma unroll
@@ -180,18 +182,18 @@ __device__ void tube_stage(const vrn::TubeArgs &a, int tb, int s, double *lds)
                 double ub = a.U[v][o + j];
                 Uo[v] = upd ? ub - coef * (F[v] - sF[3 * prev + v]) : ub;
                 if (s == 0)
-                    a.Uh[v][o + j] = Uo[v];
+                    __builtin_nontemporal_store(Uo[v], &a.Uh[v][o + j]);
                 else if (upd)
-                    a.U[v][o + j] = Uo[v];
+                    __builtin_nontemporal_store(Uo[v], &a.U[v][o + j]);
             }
             double x[3] = {a.P[0][o + j], a.P[1][o + j], a.P[2][o + j]};
             c2p1(Uo, x, g, a.n_iter, a.tol);
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 if (s == 0)
-                    a.Ph[v][o + j] = x[v];
+                    __builtin_nontemporal_store(x[v], &a.Ph[v][o + j]);
                 else
-                    a.P[v][o + j] = x[v];
+                    __builtin_nontemporal_store(x[v], &a.P[v][o + j]);
             }
         }
     }
