@@ -222,7 +222,9 @@ __global__ void k_tubes(vrn::TubeArgs a)
     }
 }
 
-// U = PtoU(P) on every cell (1dim/structs.jl:58-66); also resets the queue.
+#if !defined(VRN_FAST)
+// U = PtoU(P) on every cell (1dim/structs.jl:58-66), Uhalf = U.  Part of the
+// upload, so both arithmetic forms start from the exact-form U.
 __global__ void k_tubes_p2u(vrn::TubeArgs a, long ncell)
 {
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
@@ -236,6 +238,7 @@ __global__ void k_tubes_p2u(vrn::TubeArgs a, long ncell)
         a.Uh[2][t] = U[2];
     }
 }
+#endif
 
 // Queue reset as a kernel (graph-safe: no memset node).
 __global__ void k_queue_reset(unsigned int *q) { *q = 0u; }
