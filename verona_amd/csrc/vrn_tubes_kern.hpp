@@ -51,7 +51,8 @@ __device__ __forceinline__ void face1(const double *PL, const double *PR, double
 
 // 1dim/structs.jl:78-105: Newton on (rho, u, u^x) with F_ptoU / Jacobian and
 // the 3x3 closed-form solve of StaticArrays.jl (adjugate / det, det =
-// dot(col1, cross(col2, col3))); stops when norm(dx) < tol before applying dx.
+// dot(col1, cross(col2, col3))); stops when norm(dx) < tol before applying dx
+// (fast form: |dx|^2 < tol^2, one reciprocal each of gamma and the determinant).
 __device__ __forceinline__ void c2p1(const double *U, double *x, double g, int n_iter, double tol)
 {
     for (int it = 0; it < n_iter; it++) {
