@@ -31,7 +31,7 @@ static inline void jacobian1(const double *x, double *J, double g)
     J_(1, 3) = x[0] * x[2] / gam;
     J_(2, 1) = -(gam * gam);
     J_(2, 2) = (g - 1.0) - gam * gam * g;
-    J_(2, 3) = -2.0 * x[2] * w;
+    J_(2, 3) = ORC_MUT == ORC_MUT_JAC ? -(x[2] * w) : -2.0 * x[2] * w;
     J_(3, 1) = gam * x[2];
     J_(3, 2) = gam * x[2] * g;
     J_(3, 3) = (2.0 * (x[2] * x[2]) + 1.0) / gam * w;
@@ -58,11 +58,19 @@ static inline void sa_solve3(const double *a, const double *b, double *x)
     x[1] = ((a_(2, 3) * a_(3, 1) - a_(2, 1) * a_(3, 3)) * b[0] +
             (a_(1, 1) * a_(3, 3) - a_(1, 3) * a_(3, 1)) * b[1] +
             (a_(1, 3) * a_(2, 1) - a_(1, 1) * a_(2, 3)) * b[2]) / d;
-    x[2] = ((a_(2, 1) * a_(3, 2) - a_(2, 2) * a_(3, 1)) * b[0] +
+    x[2] = ((ORC_MUT == ORC_MUT_SOLVE3 ? a_(1, 2) * a_(2, 3) - a_(2, 2) * a_(3, 1)
+                                       : a_(2, 1) * a_(3, 2) - a_(2, 2) * a_(3, 1)) * b[0] +
             (a_(1, 2) * a_(3, 1) - a_(1, 1) * a_(3, 2)) * b[1] +
             (a_(1, 1) * a_(2, 2) - a_(1, 2) * a_(2, 1)) * b[2]) / d;
 #undef a_
 }
+
+/* Point functions of the 1D Newton, exported for their pins: F_ptoU(x) - U
+ * is the residual, J its analytic Jacobian (column-major, J[(j-1)*3+i-1] =
+ * J[i,j]), sa_solve3 the Newton step J^-1 f. */
+void orc1_f_ptou(const double *x, double g, double *f) { f_ptou(x, f, g); }
+void orc1_jacobian(const double *x, double g, double *J) { jacobian1(x, J, g); }
+void orc1_solve3(const double *A, const double *b, double *x) { sa_solve3(A, b, x); }
 
 /* 1dim/structs.jl:78-105  UtoP: 3x3 Newton per cell, warm start = P,
  * break when norm(dx) < tol before applying dx. */
@@ -80,10 +88,17 @@ void orc1_c2p(const double *U1, const double *U2, const double *U3, double *P1, 
             f[1] = f[1] - U2[i];
             f[2] = f[2] - U3[i];
             sa_solve3(J, f, d);
+            if (ORC_MUT == ORC_MUT_BREAK) {
+                x[0] = x[0] - d[0];
+                x[1] = x[1] - d[1];
+                x[2] = x[2] - d[2];
+            }
             if (sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) < tol) break;
-            x[0] = x[0] - d[0];
-            x[1] = x[1] - d[1];
-            x[2] = x[2] - d[2];
+            if (ORC_MUT != ORC_MUT_BREAK) {
+                x[0] = x[0] - d[0];
+                x[1] = x[1] - d[1];
+                x[2] = x[2] - d[2];
+            }
         }
         P1[i] = x[0];
         P2[i] = x[1];

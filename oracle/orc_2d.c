@@ -314,7 +314,8 @@ long orc2_run(double *P, int NX, int NY, const orc_mode *mode, const int *bc_typ
                 else if (st == 1)
                     stage_update2(U, Us, Us, 0.75, 0.25, dt, dx, dy, Fx, Fy, NX, NY);
                 else
-                    stage_update2(U, Us, U, 1.0 / 3.0, 2.0 / 3.0, dt, dx, dy, Fx, Fy, NX, NY);
+                    stage_update2(U, Us, U, ORC_MUT == ORC_MUT_RK3 ? 0.5 : 1.0 / 3.0,
+                                  ORC_MUT == ORC_MUT_RK3 ? 0.5 : 2.0 / 3.0, dt, dx, dy, Fx, Fy, NX, NY);
             } else {
                 if (st == 0)
                     orc2_update(U, Us, dt / 2, dx, dy, Fx, Fy, NX, NY);

@@ -152,6 +152,30 @@ static inline double res_1dw(double Qt, double W, double v2, double D, double c)
     return Qt + W - c * (W * (1.0 - v2) - D * sqrt(1.0 - v2));
 }
 
+/* Its derivative f'(W) with v^2 = S^2/W^2, 3dim/structs.jl:318-321. */
+static inline double jac_1dw(double W, double v2, double S2, double D, double c)
+{
+    double W3 = W * W * W;
+    if (ORC_MUT == ORC_MUT_F1DW) return 1.0 - c * (1.0 - v2) - c * (W * (2.0 * S2 / W3));
+    return 1.0 - c * (1.0 - v2) - c * (W * (2.0 * S2 / W3) - D * (S2 / (W3 * sqrt(1.0 - v2))));
+}
+
+/* f(W) and f'(W) of the 1DW iteration for the conserved state Uc at W,
+ * with v^2 = S^2/W^2 clamped to [0, 1 - 1e-15] as the iteration does;
+ * exported for the pins. */
+void orc3_f1dw(const double *Uc, double W, double g, double *f, double *fp)
+{
+    const double c = (g - 1.0) / g;
+    double S2 = Uc[2] * Uc[2] + Uc[3] * Uc[3] + Uc[4] * Uc[4];
+    double v2 = S2 / (W * W);
+    if (v2 < 0.0)
+        v2 = 0.0;
+    else if (v2 > 1.0 - 1e-15)
+        v2 = 1.0 - 1e-15;
+    *f = res_1dw(Uc[1], W, v2, Uc[0], c);
+    *fp = jac_1dw(W, v2, S2, Uc[0], c);
+}
+
 /* Bisection residual, 3dim/structs.jl:546-550. */
 static inline double res_bis(double Qt, double W, double S2, double D, double c)
 {
@@ -185,8 +209,7 @@ int orc3_c2p_cell(const double *Uc, double *Pc, double g, int n_iter, double tol
             else if (v2 > 1.0 - 1e-15)
                 v2 = 1.0 - 1e-15;
             double fun = res_1dw(Q_t, W, v2, D, c);
-            double W3 = W * W * W;
-            double jac = 1.0 - c * (1.0 - v2) - c * (W * (2.0 * S2 / W3) - D * (S2 / (W3 * sqrt(1.0 - v2))));
+            double jac = jac_1dw(W, v2, S2, D, c);
             double dW = fun / jac;
             double Wp = W - dW;
             if (Wp < W_min)
@@ -676,7 +699,8 @@ long orc3_run(double *P, int NX, int NY, int NZ, const orc_mode *mode, const orc
                 else if (st == 1)
                     stage_update3(U, Us, Us, 0.75, 0.25, dt, dx, dy, dz, Fx, Fy, Fz, NX, NY, NZ);
                 else
-                    stage_update3(U, Us, U, 1.0 / 3.0, 2.0 / 3.0, dt, dx, dy, dz, Fx, Fy, Fz, NX, NY, NZ);
+                    stage_update3(U, Us, U, ORC_MUT == ORC_MUT_RK3 ? 0.5 : 1.0 / 3.0,
+                                  ORC_MUT == ORC_MUT_RK3 ? 0.5 : 2.0 / 3.0, dt, dx, dy, dz, Fx, Fy, Fz, NX, NY, NZ);
             } else {
                 if (st == 0)
                     orc3_update(U, Us, dt / 2, dx, dy, dz, Fx, Fy, Fz, NX, NY, NZ);

@@ -22,6 +22,27 @@
 
 #define ORC_INLINE static inline
 
+/*
+ * Planted mistakes.  The real oracle is built with ORC_MUT undefined (0).
+ * tests/test_oracle_mutants.py builds one extra copy of the oracle per value
+ * below, each carrying exactly one plausible restatement mistake, and checks
+ * that the pin written for that function fails on it: a pin that a wrong
+ * oracle also passes would pin nothing.
+ */
+#ifndef ORC_MUT
+#define ORC_MUT 0
+#endif
+enum {
+    ORC_MUT_RK3 = 1,   /* SSP-RK3 last stage (1/2, 1/2) instead of (1/3, 2/3) */
+    ORC_MUT_PPM_C2,    /* PPM extremum limiter C2 = 1.0 instead of 1.25 */
+    ORC_MUT_PPM_EXT,   /* PPM step 4: extremum test with && instead of || */
+    ORC_MUT_JAC,       /* 1D Jacobian J[2,3] = -x3 w instead of -2 x3 w */
+    ORC_MUT_SOLVE3,    /* 3x3 solve: one adjugate entry transposed */
+    ORC_MUT_BREAK,     /* 1D Newton applies the step before the norm test */
+    ORC_MUT_F1DW,      /* 1DW f'(W): the D S^2 / (W^3 sqrt(1 - v^2)) term dropped */
+    ORC_MUT_COUNT
+};
+
 /* Julia Base: max(x::T, y::T) for IEEE floats (NaN-propagating, -0 < +0). */
 ORC_INLINE double jl_max(double x, double y)
 {
@@ -103,7 +124,7 @@ ORC_INLINE void orc_wenoz(double qm2, double qm1, double q, double qp1, double q
 ORC_INLINE void orc_ppm(double q_im2, double q_im1, double q_i, double q_ip1, double q_ip2,
                         double *lo, double *hi)
 {
-    const double C2 = 1.25;
+    const double C2 = ORC_MUT == ORC_MUT_PPM_C2 ? 1.0 : 1.25;
     double qa = q_i - q_im1;
     double qb = q_ip1 - q_i;
     double dd_im1 = 0.5 * qa + 0.5 * (q_im1 - q_im2);
@@ -173,7 +194,7 @@ ORC_INLINE void orc_ppm(double q_im2, double q_im1, double q_i, double q_ip1, do
     double tmp2_m = q_i - 2.0 * dqf_plus;
     double tmp2_p = q_i + 2.0 * dqf_minus;
 
-    if (qa_tmp <= 0.0 || qb_tmp <= 0.0) {
+    if (ORC_MUT == ORC_MUT_PPM_EXT ? (qa_tmp <= 0.0 && qb_tmp <= 0.0) : (qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
         if (rho <= 1.0 - 1.0e-12) {
             qr_i = tmp_m;
             ql_ip1 = tmp_p;

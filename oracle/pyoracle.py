@@ -38,57 +38,94 @@ class BC3(C.Structure):
     ]
 
 
+_SIG = None
+
+
+def _signatures():
+    d = C.c_double
+    p = C.POINTER(C.c_double)
+    i = C.c_int
+    L = C.c_long
+    return {
+        "orc_wenoz_v": (None, [p, L, p, p]),
+        "orc_ppm_v": (None, [p, L, p, p]),
+        "orc_minmod_v": (None, [p, L, p, p]),
+        "orc3_p2u_v": (None, [p, L, d, p]),
+        "orc3_p2f_v": (None, [p, L, d, i, p]),
+        "orc3_face_flux_v": (None, [p, p, L, d, i, i, p, p, p]),
+        "orc1_face_flux_v": (None, [p, p, L, d, i, p]),
+        "orc3_ptou": (None, [p, p, i, i, i, d]),
+        "orc3_limit": (None, [p, d, i, i, i]),
+        "orc3_fluxes": (None, [p, d, d, p, i, i, i, i, i, i]),
+        "orc3_update": (None, [p, p, d, d, d, d, p, p, p, i, i, i]),
+        "orc3_f1dw": (None, [p, d, d, p, p]),
+        "orc3_c2p_cell": (i, [p, p, d, i, d, C.POINTER(i)]),
+        "orc3_c2p": (L, [p, p, i, i, i, d, i, d, C.POINTER(L)]),
+        "orc3_c2p_batch": (L, [p, p, L, d, i, d, C.POINTER(i), C.POINTER(i)]),
+        "orc3_fill_periodic": (None, [p, i, i, i, i]),
+        "orc3_fill_outflow": (None, [p, i, i, i, i, i]),
+        "orc3_fill_nozzle": (None, [p, i, i, i, d, d, d, d, d, d, d, d]),
+        "orc3_max_rate": (d, [p, i, i, i, d, d, d, d]),
+        "orc3_run": (L, [p, i, i, i, C.POINTER(Mode), C.POINTER(BC3), i, p, p]),
+        "orc2_ptou": (None, [p, p, i, i, d]),
+        "orc2_limit": (None, [p, d, i, i]),
+        "orc2_fluxes": (None, [p, d, d, p, i, i, i, i, i]),
+        "orc2_update": (None, [p, p, d, d, d, p, p, i, i]),
+        "orc2_c2p": (L, [p, p, i, i, d, i, d, C.POINTER(L)]),
+        "orc2_fill_periodic": (None, [p, i, i, i]),
+        "orc2_fill_outflow": (None, [p, i, i, i, i]),
+        "orc2_max_rate": (d, [p, i, i, d, d, d]),
+        "orc2_run": (L, [p, i, i, C.POINTER(Mode), C.POINTER(i), i, p, p]),
+        "orc1_f_ptou": (None, [p, d, p]),
+        "orc1_jacobian": (None, [p, d, p]),
+        "orc1_solve3": (None, [p, p, p]),
+        "orc1_c2p": (None, [p, p, p, p, p, p, L, d, i, d]),
+        "orc1_run": (i, [p, p, p, L, d, d, d, i, d, i, i, i, d, p]),
+        "orc4_w0_from_prims": (None, [p, L, d, p]),
+        "orc4_solve": (L, [p, p, L, d, i, d, p, C.POINTER(i), C.POINTER(C.c_uint8)]),
+        "orc4_features": (None, [p, L, C.POINTER(C.c_float), p, p, C.POINTER(C.c_uint8)]),
+    }
+
+
+def load(path):
+    """Load an oracle build from `path` with every signature bound (the real
+    oracle, or one of the planted-mistake builds the mutant tests make)."""
+    lb = C.CDLL(path)
+    for name, (res, args) in _signatures().items():
+        f = getattr(lb, name)
+        f.restype = res
+        f.argtypes = args
+    return lb
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
-        _LIB = C.CDLL(path)
-        d = C.c_double
-        p = C.POINTER(C.c_double)
-        i = C.c_int
-        L = C.c_long
-        sig = {
-            "orc_wenoz_v": (None, [p, L, p, p]),
-            "orc_ppm_v": (None, [p, L, p, p]),
-            "orc_minmod_v": (None, [p, L, p, p]),
-            "orc3_p2u_v": (None, [p, L, d, p]),
-            "orc3_p2f_v": (None, [p, L, d, i, p]),
-            "orc3_face_flux_v": (None, [p, p, L, d, i, i, p, p, p]),
-            "orc1_face_flux_v": (None, [p, p, L, d, i, p]),
-            "orc3_ptou": (None, [p, p, i, i, i, d]),
-            "orc3_limit": (None, [p, d, i, i, i]),
-            "orc3_fluxes": (None, [p, d, d, p, i, i, i, i, i, i]),
-            "orc3_update": (None, [p, p, d, d, d, d, p, p, p, i, i, i]),
-            "orc3_c2p_cell": (i, [p, p, d, i, d, C.POINTER(i)]),
-            "orc3_c2p": (L, [p, p, i, i, i, d, i, d, C.POINTER(L)]),
-            "orc3_c2p_batch": (L, [p, p, L, d, i, d, C.POINTER(i), C.POINTER(i)]),
-            "orc3_fill_periodic": (None, [p, i, i, i, i]),
-            "orc3_fill_outflow": (None, [p, i, i, i, i, i]),
-            "orc3_fill_nozzle": (None, [p, i, i, i, d, d, d, d, d, d, d, d]),
-            "orc3_max_rate": (d, [p, i, i, i, d, d, d, d]),
-            "orc3_run": (L, [p, i, i, i, C.POINTER(Mode), C.POINTER(BC3), i, p, p]),
-            "orc2_ptou": (None, [p, p, i, i, d]),
-            "orc2_limit": (None, [p, d, i, i]),
-            "orc2_fluxes": (None, [p, d, d, p, i, i, i, i, i]),
-            "orc2_update": (None, [p, p, d, d, d, p, p, i, i]),
-            "orc2_c2p": (L, [p, p, i, i, d, i, d, C.POINTER(L)]),
-            "orc2_fill_periodic": (None, [p, i, i, i]),
-            "orc2_fill_outflow": (None, [p, i, i, i, i]),
-            "orc2_max_rate": (d, [p, i, i, d, d, d]),
-            "orc2_run": (L, [p, i, i, C.POINTER(Mode), C.POINTER(i), i, p, p]),
-            "orc1_c2p": (None, [p, p, p, p, p, p, L, d, i, d]),
-            "orc1_run": (i, [p, p, p, L, d, d, d, i, d, i, i, i, d, p]),
-            "orc4_w0_from_prims": (None, [p, L, d, p]),
-            "orc4_solve": (L, [p, p, L, d, i, d, p, C.POINTER(i), C.POINTER(C.c_uint8)]),
-            "orc4_features": (None, [p, L, C.POINTER(C.c_float), p, p, C.POINTER(C.c_uint8)]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(_LIB, name)
-            f.restype = res
-            f.argtypes = args
+        _LIB = load(path)
     return _LIB
+
+
+class use:
+    """Context manager: route every function of this module through another
+    oracle build (a loaded library from load())."""
+
+    def __init__(self, other):
+        self.other = other
+
+    def __enter__(self):
+        global _LIB
+        lib()
+        self.saved = _LIB
+        _LIB = self.other
+        return self
+
+    def __exit__(self, *exc):
+        global _LIB
+        _LIB = self.saved
+        return False
 
 
 def _p(a):
@@ -155,6 +192,40 @@ def face_flux1(PL, PR, gamma, riemann="hll"):
     F = np.empty_like(PL)
     lib().orc1_face_flux_v(_p(PL), _p(PR), len(PL), gamma, RIEMANN[riemann], _p(F))
     return F
+
+
+def f1dw3(U, W, gamma):
+    """1DW residual f(W) and the iteration's analytic f'(W) for one conserved state."""
+    U = _f64(U).reshape(5)
+    f = C.c_double(0)
+    fp = C.c_double(0)
+    lib().orc3_f1dw(_p(U), float(W), gamma, C.byref(f), C.byref(fp))
+    return f.value, fp.value
+
+
+def f_ptou1(x, gamma):
+    """F_ptoU of the 1D Newton: conserved (rho Gamma, T^t_t, T^t_x) of P = (rho, u, u^x)."""
+    x = _f64(x).reshape(3)
+    f = np.empty(3)
+    lib().orc1_f_ptou(_p(x), gamma, _p(f))
+    return f
+
+
+def jacobian1(x, gamma):
+    """Analytic Jacobian of F_ptoU as the 1D Newton forms it, J[i, j] = dF_i/dx_j."""
+    x = _f64(x).reshape(3)
+    J = np.empty(9)
+    lib().orc1_jacobian(_p(x), gamma, _p(J))
+    return J.reshape(3, 3).T.copy()  # stored column-major
+
+
+def solve3(A, b):
+    """The 1D Newton's 3x3 solve of A x = b (A given as a 3x3 row-major array)."""
+    Acm = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T).reshape(9)
+    b = _f64(b).reshape(3)
+    x = np.empty(3)
+    lib().orc1_solve3(_p(Acm), _p(b), _p(x))
+    return x
 
 
 # ---------------------------------------------------------------- 3D

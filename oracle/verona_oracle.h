@@ -49,6 +49,7 @@ void orc3_fluxes(const double *P, double g, double floor_, double *F, int dim, i
                  int NZ, int rec, int riemann);
 void orc3_update(const double *U, double *Ub, double dt, double dx, double dy, double dz,
                  const double *Fx, const double *Fy, const double *Fz, int NX, int NY, int NZ);
+void orc3_f1dw(const double *Uc, double W, double g, double *f, double *fp);
 int orc3_c2p_cell(const double *Uc, double *Pc, double g, int n_iter, double tol, int *iters);
 long orc3_c2p(const double *U, double *P, int NX, int NY, int NZ, double g, int n_iter,
               double tol, long *method_counts);
@@ -79,6 +80,9 @@ long orc2_run(double *P, int NX, int NY, const orc_mode *mode, const int *bc_typ
               double *t_out, double *dt_last);
 
 /* 1D (SoA, no ghosts: rho[N], u[N], ux[N]) */
+void orc1_f_ptou(const double *x, double g, double *f);
+void orc1_jacobian(const double *x, double g, double *J);
+void orc1_solve3(const double *A, const double *b, double *x);
 void orc1_c2p(const double *U1, const double *U2, const double *U3, double *P1, double *P2,
               double *P3, long N, double g, int n_iter, double tol);
 int orc1_run(double *P1, double *P2, double *P3, long N, double dt, double dx, double T,

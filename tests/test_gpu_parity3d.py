@@ -230,14 +230,17 @@ def test_fp32_form_error_is_bounded(oracle):
     assert max(errs) < 1e-5
 
 
-def test_jet_nozzle_cfl_ext_mode(oracle):
+@pytest.mark.parametrize("precision", ["fp64", "fp64fast"])
+def test_jet_nozzle_cfl_ext_mode(oracle, precision):
     """The jet workload at small size: nozzle inflow on z-low with a per-step
-    precession angle, outflow elsewhere, HLLC + SSP-RK3 + CFL dt."""
+    precession angle, outflow elsewhere, HLLC + SSP-RK3 + CFL dt, in both FP64
+    forms (fp64fast is the form of the benchmark line): relative L1 per
+    variable <= 1e-12 after 6 steps and t within 1e-13 of the oracle's."""
     import math
 
     from verona_amd import DeviceGrid, host_array
     from verona_amd.bc import uniform_stream
-    progress("test_jet_nozzle_cfl_ext_mode 40x40x32")
+    progress(f"test_jet_nozzle_cfl_ext_mode 40x40x32 {precision}")
     n, nzc = 40, 32
     P = host_array(3, n, n, nzc)
     P[0] = 1.0
@@ -253,7 +256,7 @@ def test_jet_nozzle_cfl_ext_mode(oracle):
         dev.fill_ghosts(bc)
         dev.prim_to_cons()
         dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50,
-                     tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx, bc=bc)
+                     tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx, bc=bc, precision=precision)
         tg = sum(dev.step(phi=phis[s], want_dt=True) for s in range(6))
         Pg = dev.download("P")
         nbad, _ = dev.counters()
@@ -267,7 +270,7 @@ def test_jet_nozzle_cfl_ext_mode(oracle):
     progress(f"  per-var rel L1 {['%.2e' % e for e in errs]}, t {tg!r} vs {to!r}, max u_z {Pg[4].max():.3f}")
     assert nbad == nbo == 0
     assert max(errs) <= TOL
-    assert abs(tg - to) <= 1e-14 * to
+    assert abs(tg - to) <= (1e-14 if precision == "fp64" else 1e-13) * to
     assert Pg[4, 3:-3, 3:-3, 3].max() > 0.5  # the jet has entered the grid
 
 

@@ -407,3 +407,43 @@ def test_near_vacuum_front_is_ill_conditioned_2d(oracle):
     s = spread(True)
     assert s[0] < 1e-12 and max(s[2], s[3]) > 1e-8
     assert max(spread(False)) < 1e-14
+
+
+# ----------------------------------------------------------------- functions pinned by
+# convergence orders, finite differences and exact arithmetic (tests/pin_checks.py;
+# tests/test_oracle_mutants.py shows each check failing on a planted mistake)
+@pytest.mark.parametrize("dims", [2, 3])
+def test_ssp_rk3_third_order_in_time(oracle, dims):
+    import pin_checks
+    rates = pin_checks.rk3_temporal_order(oracle, dims)
+    assert min(rates) > 2.7
+
+
+def test_ppm_fourth_order_on_smooth_data(oracle):
+    import pin_checks
+    pin_checks.ppm_convergence_order(oracle)
+
+
+def test_ppm_leaves_smooth_extrema_unclipped(oracle):
+    import pin_checks
+    pin_checks.ppm_smooth_extremum(oracle)
+
+
+def test_1d_jacobian_matches_finite_differences(oracle):
+    import pin_checks
+    pin_checks.jacobian1_fd(oracle)
+
+
+def test_1d_3x3_solve_is_exact_on_integer_systems(oracle):
+    import pin_checks
+    pin_checks.solve3_exact(oracle)
+
+
+def test_1d_newton_breaks_before_update(oracle):
+    import pin_checks
+    pin_checks.newton1_break_before_update(oracle)
+
+
+def test_1dw_derivative_matches_finite_differences(oracle):
+    import pin_checks
+    pin_checks.f1dw_derivative_fd(oracle)

@@ -186,6 +186,8 @@ __global__ __launch_bounds__(256) void k_rate(Geo G, const double *P, double g, 
     rate_reduce(st, r);
 }
 
+__global__ void k_reset_rate(DevState *st) { st->rate_bits = 0ull; }
+
 // dt = cfl / max(interior rate, boundary inflow rate); resets the accumulator.
 __global__ void k_finalize_dt(DevState *st, double cfl, double extra_rate)
 {
@@ -406,7 +408,12 @@ struct vrn_ctx {
     // nozzle
     double noz_r = 0, noz_dx = 1, noz_dy = 1, noz_rho = 0, noz_u = 0, noz_umag = 0, noz_eps = 0;
     bool have_noz = false;
-    bool first_cfl = true;
+    // rate_bits holds the max signal rate of the current P for spacings
+    // rate_dx only while rate_ok: a CFL step leaves it there (its last stage
+    // reduces the new state's rate); anything else that writes P, a fixed-dt
+    // step or new spacings invalidate it, and the next CFL step recomputes it.
+    bool rate_ok = false;
+    double rate_dx[3] = {0, 0, 0};
     double dxs[3] = {1, 1, 1};
     // tuning
     int rows_per_wave = 8;
@@ -695,7 +702,7 @@ int vrn_upload(vrn_ctx *c, int slot, const double *host)
     else
         hipLaunchKernelGGL(k_aos_to_soa<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, c->stg, d);
     HIPCHK(hipGetLastError());
-    if (slot == VRN_SLOT_P) c->first_cfl = true;
+    if (slot == VRN_SLOT_P) c->rate_ok = false;
     HIPCHK(hipStreamSynchronize(c->s));
     return 0;
 }
@@ -727,6 +734,7 @@ int vrn_copy(vrn_ctx *c, int dst_slot, int src_slot)
     double *d = slot_ptr(c, dst_slot), *s_ = slot_ptr(c, src_slot);
     if (!d || !s_ || d == s_) return fail(VRN_E_ARG, "bad slots %d <- %d", dst_slot, src_slot);
     HIPCHK(hipMemcpyAsync(d, s_, sizeof(double) * (size_t)c->NV * c->G.ps, hipMemcpyDeviceToDevice, c->s));
+    if (dst_slot == VRN_SLOT_P) c->rate_ok = false;
     return 0;
 }
 
@@ -758,6 +766,7 @@ int vrn_limit(vrn_ctx *c, double floor_)
     else
         hipLaunchKernelGGL(k_ref_limit<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, c->G, c->P, floor_);
     HIPCHK(hipGetLastError());
+    c->rate_ok = false;
     return 0;
 }
 
@@ -834,6 +843,7 @@ int vrn_cons_to_prim(vrn_ctx *c, int src_slot, int n_iter, double tol)
     else
         hipLaunchKernelGGL(k_ref_c2p<4>, dim3(grid_blocks(n, 64)), dim3(64), 0, c->s, G, slot_ptr(c, src_slot), c->P, c->gamma, n_iter, tol, c->st);
     HIPCHK(hipGetLastError());
+    c->rate_ok = false;
     return 0;
 }
 
@@ -961,17 +971,18 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     const Geo &G = c->G;
     const bool cflm = p->cfl > 0.0;
     // dt: fixed, or CFL from the state at the step start
+    const double dz_rate = c->ndim == 3 ? p->dz : 1.0;
     if (cflm) {
-        if (c->first_cfl) {
+        if (!c->rate_ok || c->rate_dx[0] != p->dx || c->rate_dx[1] != p->dy || c->rate_dx[2] != dz_rate) {
             Timed tm(c, KN_CFL);
+            hipLaunchKernelGGL(k_reset_rate, dim3(1), dim3(1), 0, c->s, c->st);
             if (c->NV == 5)
                 hipLaunchKernelGGL(k_rate<5>, dim3(2048), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0 / p->dz, c->st);
             else
                 hipLaunchKernelGGL(k_rate<4>, dim3(2048), dim3(256), 0, c->s, G, c->P, c->gamma, 1.0 / p->dx, 1.0 / p->dy, 1.0, c->st);
-            c->first_cfl = false;
         }
-        // the rate of the cells on the path is in rate_bits (from the last
-        // stage of the previous step, or from k_rate); the nozzle's inflow
+        // the rate of the current P is in rate_bits (from the last stage of
+        // the previous CFL step, or from k_rate just now); the nozzle's inflow
         // state enters through the ghost faces and is added here.
         double extra = 0.0;
         if (p->bc[4] == VRN_BC_NOZZLE) {
@@ -1055,6 +1066,11 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
         HIPCHK(hipGetLastError());
         std::swap(c->P, c->Pn);
     }
+    // a CFL step's last stage left the new state's rate in rate_bits
+    c->rate_ok = cflm;
+    c->rate_dx[0] = p->dx;
+    c->rate_dx[1] = p->dy;
+    c->rate_dx[2] = dz_rate;
     if (dt_used) {
         if (cflm) {
             HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->s));

@@ -141,6 +141,7 @@ def harm_hll(dev, ndim, comm, P_host_view, dt, grid, Tmax, eos, drops, floor, ou
     for _ in range(i):
         thres += drops
     t0 = time.time()
+    steps_at_print = steps
     ncell = int(np.prod([s for s in dev.shape[1:]]))
     while t < Tmax:
         if max_steps is not None and steps >= max_steps:
@@ -163,8 +164,12 @@ def harm_hll(dev, ndim, comm, P_host_view, dt, grid, Tmax, eos, drops, floor, ou
                 dev.sync()
                 el = time.time() - t0
                 print(f"{t:.3g} elapsed: {el:.3g} s")
-                print(f"speed: {ncell * drops / dt / el * 1e-6:.3g} zones [10^6/s]")
+                # the reference prints cells * drops / dt / elapsed (src/3dim/algos.jl:344-352),
+                # i.e. cells x steps taken since the last print for a fixed dt; the step
+                # count is what it means, and is also defined in CFL mode (dt = 0)
+                print(f"speed: {ncell * (steps - steps_at_print) / el * 1e-6:.3g} zones [10^6/s]")
                 t0 = time.time()
+                steps_at_print = steps
             if out_dir is not None:
                 save_dump(os.path.join(out_dir, f"dump{i}.npz"), dev.download("P"), t, grid, steps)
     dev.sync()
