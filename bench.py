@@ -121,16 +121,29 @@ def run_kh(n=2048, steps=40, warmup=3, precision="fp64fast"):
             "ms_per_step": el / steps * 1e3, "n_unrecovered": nbad}
 
 
-def run_tubes(ntubes=4096, steps=500, precision="fp64", keep=False):
+def _tube_subset(sel, lengths, rho, u, ux, dt, dx):
+    """The inputs of the tubes sel (indices into the batch), as their own batch."""
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    idx = np.concatenate([np.arange(off[t], off[t + 1]) for t in sel])
+    return lengths[sel], rho[idx], u[idx], ux[idx], dt[sel], dx[sel]
+
+
+def run_tubes(ntubes=4096, steps=500, precision="fp64", keep=False, subset=None):
     """Batched uneven 1D shock tubes: 4096 tubes, lengths U{256..8192} (SplitMix64 seed
     42), PPM + HLLC, RK2, dt = 0.4 dx per tube, one persistent hipGraph launch with the
-    autotuned launch shape.  value = cells x steps / s of the timed launch."""
+    autotuned launch shape.  value = cells x steps / s of the timed launch.  subset:
+    run only these tubes of the batch, as a batch of their own.  The autotuner
+    probes the state half-way through the run (steps / 2 untimed steps first)."""
     from verona_amd.tubes import Tubes, shock_tube_batch
-    lengths, rho, u, ux, dt, dx = shock_tube_batch(ntubes, 256, 8192, seed=42)
+    batch = shock_tube_batch(ntubes, 256, 8192, seed=42)
+    if subset is not None:
+        batch = _tube_subset(subset, *batch)
+    lengths, rho, u, ux, dt, dx = batch
     with Tubes(lengths) as tb:
         tb.upload(rho, u, ux, dt, dx)
         tb.set_method("ppm", "hllc", 50, 1e-10)
         tb.set_precision(precision)
+        tb.run(steps // 2, 256, 4 * tb.cu_count)  # a developed state for the autotuner
         block, grid = tb.autotune(probe_steps=2)
         tb.upload(rho, u, ux, dt, dx)
         tb.run(2, block, grid)  # graph capture + warm-up
@@ -141,33 +154,50 @@ def run_tubes(ntubes=4096, steps=500, precision="fp64", keep=False):
     off = np.concatenate([[0], np.cumsum(lengths)])
     # Verona1D's con2prim has no floors: tubes whose random states it cannot
     # recover end with NaN markers (the oracle marks the same cells)
-    bad = int(sum(not np.isfinite(r[off[t]:off[t + 1]]).all() for t in range(ntubes)))
-    out = {"what": f"{ntubes} uneven 1D tubes ({cells} cells), PPM+HLLC, RK2, {steps} steps, "
+    finite = np.array([bool(np.isfinite(r[off[t]:off[t + 1]]).all() and np.isfinite(uu[off[t]:off[t + 1]]).all()
+                            and np.isfinite(vx[off[t]:off[t + 1]]).all()) for t in range(len(lengths))])
+    out = {"what": f"{len(lengths)} uneven 1D tubes ({cells} cells), PPM+HLLC, RK2, {steps} steps, "
                    "one persistent hipGraph launch", "form": precision, "value": cells * steps / (ms * 1e-3),
            "unit": "cell-steps/s", "ms": ms, "launch_shape": [block, grid],
-           "tubes_with_nan_markers": bad}
+           "tubes_with_nan_markers": int((~finite).sum()), "_finite": finite}
     if keep:
         out["_state"] = (off, np.stack([r, uu, vx]))
     return out
 
 
 def run_tubes_both(ntubes=4096, steps=500):
-    """The tubes leg in the exact FP64 form (the line's value) and in the fast
-    FP64 form, with the fast form's relative L1 difference from the exact one
-    over the tubes both leave finite."""
+    """The tubes leg in the exact FP64 form and in the fast FP64 form.  The
+    line's value is the rate over the tubes that stay finite (the batch minus
+    the tubes Verona1D's floor-free Newton leaves with NaN markers, re-run as a
+    batch of their own, so no cell skips its con2prim); the all-tubes rates
+    are kept beside it.  The fast form's relative L1 difference from the exact
+    one is taken over the tubes both leave finite."""
     ex = run_tubes(ntubes, steps, "fp64", keep=True)
     fa = run_tubes(ntubes, steps, "fp64fast", keep=True)
     off, A = ex.pop("_state")
     _, B = fa.pop("_state")
-    ok = np.ones(len(off) - 1, bool)
-    for t in range(len(off) - 1):
-        ok[t] = np.isfinite(A[:, off[t]:off[t + 1]]).all() and np.isfinite(B[:, off[t]:off[t + 1]]).all()
+    fin_ex, fin_fa = ex.pop("_finite"), fa.pop("_finite")
+    ok = fin_ex & fin_fa
     m = np.repeat(ok, np.diff(off))
-    fa["rel_l1_vs_exact"] = [float(np.abs(B[v][m] - A[v][m]).sum() / np.abs(A[v][m]).sum()) for v in range(3)]
-    fa["tubes_compared"] = int(ok.sum())
-    ex["fast_form"] = {k: fa[k] for k in ("value", "unit", "ms", "launch_shape", "tubes_with_nan_markers",
-                                          "rel_l1_vs_exact", "tubes_compared")}
-    return ex
+    rel = [float(np.abs(B[v][m] - A[v][m]).sum() / np.abs(A[v][m]).sum()) for v in range(3)]
+    sel = np.flatnonzero(fin_ex)
+    exf = run_tubes(ntubes, steps, "fp64", subset=sel)
+    faf = run_tubes(ntubes, steps, "fp64fast", subset=sel)
+    for d in (exf, faf):
+        d.pop("_finite")
+    keys = ("value", "unit", "ms", "launch_shape", "tubes_with_nan_markers")
+    out = {"what": f"{len(sel)} of {ntubes} uneven 1D tubes (those Verona1D's con2prim keeps finite), "
+                   f"PPM+HLLC, RK2, {steps} steps, one persistent hipGraph launch",
+           "form": "fp64", "value": exf["value"], "unit": "cell-steps/s", "ms": exf["ms"],
+           "launch_shape": exf["launch_shape"], "tubes": int(len(sel)),
+           "tubes_with_nan_markers": exf["tubes_with_nan_markers"],
+           "all_tubes": {"what": ex["what"], **{k: ex[k] for k in keys}},
+           "fast_form": {"value": faf["value"], "unit": "cell-steps/s", "ms": faf["ms"],
+                         "launch_shape": faf["launch_shape"],
+                         "tubes_with_nan_markers": faf["tubes_with_nan_markers"],
+                         "all_tubes": {k: fa[k] for k in keys},
+                         "rel_l1_vs_exact": rel, "tubes_compared": int(ok.sum())}}
+    return out
 
 
 def fp32_error(n, steps):
@@ -341,16 +371,47 @@ def copy_bandwidth(n, reps=5):
     return 2.0 * nbytes * reps / el / 1e9
 
 
-def measured_traffic(kernel, n, precision):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    passes (copy-calibrated), when they were taken on this workload and form; else None."""
+def _traffic_profile(n, precision):
+    """The committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (copy-calibrated) of the
+    jet's three sweeps, when they were taken on this workload and form; else None."""
     path = os.path.join(ROOT, "profiles", "jet384_rk3_hbm_traffic.json")
     if n != 384 or precision != "fp64fast" or not os.path.exists(path):
         return None
     with open(path) as f:
-        d = json.load(f)
-    k = d.get("kernels", {}).get(kernel)
+        return json.load(f).get("kernels", {})
+
+
+def measured_traffic(kernel, n, precision):
+    """HBM bytes per launch of `kernel` from the committed counter passes, or None."""
+    k = (_traffic_profile(n, precision) or {}).get(kernel)
     return None if k is None else float(k["bytes"])
+
+
+def measured_step_traffic(n, precision):
+    """Measured HBM bytes per zone-update of the whole step: the three sweeps'
+    calibrated bytes per interior cell per launch, each launched once per RK
+    stage, times 3 stages (the rescue and ghost passes are < 1 % of it)."""
+    kern = _traffic_profile(n, precision)
+    if not kern or not all(k in kern for k in ("xsweep", "ysweep", "zsweep_c2p")):
+        return None
+    return 3.0 * sum(kern[k]["per_cell"] for k in ("xsweep", "ysweep", "zsweep_c2p"))
+
+
+def valu_issue_ceiling(n, precision):
+    """Zone-updates/s the step's instruction count allows at full VALU issue:
+    the three sweeps' VALU lane-instructions per interior cell per launch
+    (committed rocprofv3 SQ_INSTS_VALU pass), three launches each per step,
+    against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction."""
+    path = os.path.join(ROOT, "profiles", "jet384_sweeps_instruction_mix.json")
+    if n != 384 or precision != "fp64fast" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        mix = json.load(f)["per_cell_lane_instructions"]
+    if not all(k in mix for k in ("xsweep", "ysweep", "zsweep_c2p")):
+        return None
+    lane_ops = 3.0 * sum(mix[k]["SQ_INSTS_VALU"] for k in ("xsweep", "ysweep", "zsweep_c2p"))
+    return {"valu_lane_ops_per_zone_update": lane_ops,
+            "zone_updates_per_s": 256 * 4 * 2.4e9 / 4 * 64 / lane_ops}
 
 
 def fp64_roofline(kernel, cells, mean_ms, n, precision):
@@ -428,10 +489,16 @@ def main():
     tot = {k: c * m for k, (c, m) in rep.items()}
     dom = max(tot, key=tot.get)
     cnt, mean_ms = rep[dom]
-    # per-launch compulsory bytes of the dominant kernel's I/O contract (per interior cell)
-    per_cell = {"zsweep_c2p": 5 * 40, "xsweep": 3 * 40, "ysweep": 3 * 40}.get(dom, 4 * 40)
+    # Algorithmic bytes per launch of the dominant kernel, per interior cell.
+    # The z sweep closes each RK stage, so it carries the stage's compulsory
+    # traffic: 560 B per zone-update over 3 stages (stage 1: read P, U^n, write
+    # U, P = 160 B; stages 2-3 also read U^n = 200 B; mean 560/3 B per launch).
+    # The x and y sweeps' own contract is read P, read U_b/acc, write acc = 120 B.
+    per_cell = {"zsweep_c2p": BYTES_PER_ZU["rk3"] / 3.0, "xsweep": 3 * 40, "ysweep": 3 * 40}.get(dom, 4 * 40)
     ach = per_cell * cells / (mean_ms * 1e-3) / 1e9
     step_gbs = BYTES_PER_ZU["rk3"] * value / 1e9
+    step_traffic = measured_step_traffic(n, args.precision)
+    ceiling = valu_issue_ceiling(n, args.precision)
     copy_gbs = copy_bandwidth(n)
     out = {
         "metric": "zone-updates/s (3D WENOZ+HLLC, 1 GPU); con2prim cells/s learned vs Newton",
@@ -457,10 +524,20 @@ def main():
                      "traffic": measured_traffic(dom, n, args.precision),
                      "traffic_source": "profiles/jet384_rk3_hbm_traffic.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)",
                      "bytes_per_cell_per_launch": per_cell, "mean_launch_ms": mean_ms,
-                     "launches": cnt, "algorithmic_bytes": per_cell * cells},
+                     "launches": cnt, "algorithmic_bytes": per_cell * cells,
+                     # the metric itself: the whole step at 560 B per zone-update
+                     "step": {"bytes_per_zone_update": BYTES_PER_ZU["rk3"], "achieved": step_gbs,
+                              "frac": step_gbs / HBM_PEAK_GBS,
+                              "measured_traffic_bytes_per_zone_update": step_traffic,
+                              "traffic_over_algorithmic": (None if step_traffic is None
+                                                           else step_traffic / BYTES_PER_ZU["rk3"]),
+                              "hbm_ceiling_zone_updates_per_s": HBM_PEAK_GBS * 1e9 / BYTES_PER_ZU["rk3"],
+                              "valu_issue_ceiling": ceiling}},
         "roofline_fp64": fp64_roofline(dom, cells, mean_ms, n, args.precision),
         "roofline_step": {"bytes_per_zone_update": BYTES_PER_ZU["rk3"], "achieved": step_gbs,
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
+                          "measured_traffic_bytes_per_zone_update": step_traffic,
+                          "valu_issue_ceiling": ceiling},
         "kernels_ms": {k: {"launches": c, "mean_ms": m} for k, (c, m) in rep.items()},
         "n_unrecovered": nbad,
         "n_rescued": nresc,
