@@ -80,6 +80,25 @@ int vrn_c2p_upload(vrn_c2p *h, int64_t n, const double *U, const double *P_prev)
  * relative stopping tolerance rtol.  stats may be NULL. */
 int vrn_c2p_solve(vrn_c2p *h, int guess, int n_iter, double rtol, vrn_c2p_stats *stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the solve of vrn_c2p_solve (same kernels, same results
+ * bit for bit) on caller-owned device arrays, with no handle and no
+ * allocation.  U, P_prev (VRN_GUESS_WARM only, else may be NULL) and P are
+ * device pointers to 5 planes of n doubles each (structure of arrays: variable
+ * v of cell i at ptr[v * n + i]).  guess is VRN_GUESS_GENERIC, _WARM or
+ * _ANALYTIC (the learned guess needs a handle's network).  Work is enqueued
+ * on `stream` (a hipStream_t; NULL = the null stream) and the call returns;
+ * with stats != NULL it waits for the stream and fills them (ms = 0: not
+ * timed).  The working memory is a fixed device array inside the library,
+ * sized for n <= VRN_C2P_NOALLOC_MAX_CELLS (about 19 MB on every GPU the
+ * library runs on, held until the process ends); larger n is rejected.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams or host threads) race and must be serialised by the caller. */
+#define VRN_C2P_NOALLOC_MAX_CELLS (1 << 20)
+int vrn_c2p_solve_noalloc(int64_t n, const double *U, const double *P_prev, double *P, double gamma,
+                          int guess, int n_iter, double rtol, void *stream, vrn_c2p_stats *stats);
+
 /* Results of the last solve: primitives [n][5], the W each cell started
  * from, and its iteration count (capped at 255).  Any pointer may be NULL. */
 int vrn_c2p_download(vrn_c2p *h, double *P, double *W0, uint8_t *iters);

@@ -52,6 +52,7 @@ C2P_SIGNATURES = {
     "vrn_c2p_upload": (_I, [_P, _I64, _DP, _DP]),
     "vrn_c2p_solve": (_I, [_P, _I, _I, _D, C.POINTER(C2PStats)]),
     "vrn_c2p_download": (_I, [_P, _DP, _DP, _U8P]),
+    "vrn_c2p_solve_noalloc": (_I, [_I64, _P, _P, _P, _D, _I, _I, _D, _P, C.POINTER(C2PStats)]),
     "vrn_mlp_set": (_I, [_P, _FP]),
     "vrn_mlp_get": (_I, [_P, _FP]),
     "vrn_mlp_features": (_I, [_P, _I64, _DP, _DP, _FP, _FP, _U8P]),
@@ -88,6 +89,19 @@ def stats_dict(st):
             "iters_max": int(st.iters_max), "wave_max_mean": st.wave_max_sum / max(int(st.n_waves), 1),
             "ms": float(st.ms), "cells_per_s": st.n / (st.ms * 1e-3) if st.ms > 0 else None,
             "hist": hist.tolist()}
+
+
+def solve_noalloc(n, U_dev, P_dev, gamma=4.0 / 3.0, guess="analytic", P_prev_dev=None, n_iter=100,
+                  rtol=1e-12, stream=None, stats=True):
+    """vrn_c2p_solve_noalloc on caller-owned device arrays (addresses as
+    integers, e.g. torch's ``data_ptr()``): 5 planes of n doubles each for U,
+    P_prev and P.  Allocates nothing; with stats, waits for the stream."""
+    st = C2PStats() if stats else None
+    _check(_lib.load().vrn_c2p_solve_noalloc(int(n), U_dev, P_prev_dev, P_dev, float(gamma), GUESS[guess],
+                                             int(n_iter), float(rtol), stream,
+                                             C.byref(st) if stats else None),
+           "vrn_c2p_solve_noalloc")
+    return stats_dict(st) if stats else None
 
 
 class Con2PrimBatch:

@@ -42,6 +42,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1373,3 +1374,121 @@ int vrn_mlp_train(vrn_c2p *h, int64_t n, const double *U, const double *P, const
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- no-allocation solve
+// vrn_c2p_solve_noalloc: the solve of vrn_c2p_solve on caller-owned device
+// arrays, with its working memory (counters, per-block counter rows, rescue
+// list) in the file-scope array below instead of a handle's allocations.
+// Every GPU the library runs on holds one copy of it until the process ends;
+// the host takes its device address once per device with
+// hipGetSymbolAddress, since the array's name in host code is only a
+// host-side stand-in.
+namespace {
+constexpr size_t NA_CNT_BYTES = 1024;  // Counters (576 B), padded
+constexpr size_t NA_PART_BYTES = sizeof(unsigned long long) * PART_STRIDE * MAX_SOLVE_GRID;
+constexpr size_t NA_RESCUE_BYTES = sizeof(Rescue) * (size_t)VRN_C2P_NOALLOC_MAX_CELLS;
+constexpr size_t NA_SCRATCH_BYTES = NA_CNT_BYTES + NA_PART_BYTES + NA_RESCUE_BYTES;
+static_assert(sizeof(Counters) <= NA_CNT_BYTES, "counter block");
+static_assert(NA_PART_BYTES % 256 == 0, "rescue list alignment");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+unsigned char *na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int noalloc_scratch(unsigned char **out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(-1, "vrn_c2p_solve_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(na_mutex);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(-2, "vrn_c2p_solve_noalloc: no device address for the scratch array");
+        na_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = na_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int vrn_c2p_solve_noalloc(int64_t n, const double *U, const double *P_prev, double *P, double gamma,
+                                     int guess, int n_iter, double rtol, void *stream, vrn_c2p_stats *stats)
+{
+    if (n < 1 || n > VRN_C2P_NOALLOC_MAX_CELLS)
+        return fail(-1, "vrn_c2p_solve_noalloc: n = %lld outside [1, %d] (the scratch array's capacity)",
+                    (long long)n, VRN_C2P_NOALLOC_MAX_CELLS);
+    if (!U || !P) return fail(-1, "vrn_c2p_solve_noalloc: NULL U or P");
+    if (!(gamma > 1.0 && gamma <= 2.0)) return fail(-1, "vrn_c2p_solve_noalloc: gamma %g outside (1, 2]", gamma);
+    if (guess != VRN_GUESS_GENERIC && guess != VRN_GUESS_WARM && guess != VRN_GUESS_ANALYTIC)
+        return fail(-1, "vrn_c2p_solve_noalloc: guess %d not available (GENERIC, WARM or ANALYTIC)", guess);
+    if (guess == VRN_GUESS_WARM && !P_prev) return fail(-1, "vrn_c2p_solve_noalloc: warm guess needs P_prev");
+    if (n_iter < 1 || n_iter > 10000) return fail(-1, "vrn_c2p_solve_noalloc: n_iter %d out of range", n_iter);
+    if (!(rtol > 0.0 && rtol < 1.0)) return fail(-1, "vrn_c2p_solve_noalloc: rtol %g out of range", rtol);
+    unsigned char *base = nullptr;
+    if (int rc = noalloc_scratch(&base)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Counters *cnt = reinterpret_cast<Counters *>(base);
+    SolveArgs a;
+    a.n = n;
+    a.U = U;
+    a.Pprev = P_prev;
+    a.P = P;
+    a.W0 = nullptr;
+    a.iters = nullptr;
+    a.cnt = cnt;
+    a.part = reinterpret_cast<unsigned long long *>(base + NA_CNT_BYTES);
+    a.rescue = reinterpret_cast<Rescue *>(base + NA_CNT_BYTES + NA_PART_BYTES);
+    a.g = gamma;
+    a.rtol = rtol;
+    a.n_iter = n_iter;
+    a.record = 0;
+    a.net = nullptr;
+    // the counters keep the previous call's values: cleared by a kernel
+    // (no memset node, so the sequence can also be captured into a graph)
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(128), 0, s, cnt);
+    const int grid = grid_for((n + 63) / 64, 4, MAX_SOLVE_GRID);
+    const int rgrid = 1024;
+    switch (guess) {
+    case VRN_GUESS_GENERIC:
+        hipLaunchKernelGGL(k_solve<VRN_GUESS_GENERIC>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_reduce_counters, dim3(PART_STRIDE), dim3(256), 0, s,
+                           (const unsigned long long *)a.part, grid, cnt);
+        hipLaunchKernelGGL(k_rescue<VRN_GUESS_GENERIC>, dim3(rgrid), dim3(256), 0, s, a);
+        break;
+    case VRN_GUESS_WARM:
+        hipLaunchKernelGGL(k_solve<VRN_GUESS_WARM>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_reduce_counters, dim3(PART_STRIDE), dim3(256), 0, s,
+                           (const unsigned long long *)a.part, grid, cnt);
+        hipLaunchKernelGGL(k_rescue<VRN_GUESS_WARM>, dim3(rgrid), dim3(256), 0, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_solve<VRN_GUESS_ANALYTIC>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_reduce_counters, dim3(PART_STRIDE), dim3(256), 0, s,
+                           (const unsigned long long *)a.part, grid, cnt);
+        hipLaunchKernelGGL(k_rescue<VRN_GUESS_ANALYTIC>, dim3(rgrid), dim3(256), 0, s, a);
+        break;
+    }
+    HIPCHK(hipGetLastError());
+    if (stats) {
+        Counters c;
+        HIPCHK(hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        stats->n = n;
+        stats->n_unrecovered = (int64_t)c.n_unrecovered;
+        stats->n_fallback = (int64_t)c.n_fallback;
+        stats->n_guess_bad = (int64_t)c.n_guess_bad;
+        stats->iters_sum = (int64_t)c.iters_sum;
+        stats->iters_max = (int64_t)c.iters_max;
+        stats->wave_max_sum = (int64_t)c.wave_max_sum;
+        stats->n_waves = (int64_t)c.n_waves;
+        for (int b = 0; b < 64; b++) stats->hist[b] = (int64_t)c.hist[b];
+        stats->ms = 0.0;
+    }
+    return 0;
+}
