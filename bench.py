@@ -95,30 +95,105 @@ def run_gpu(n, steps, warmup, *, ref, precision="fp64fast", timing=True):
     return el, rep, nbad, nresc
 
 
-def run_kh(n=2048, steps=40, warmup=3, precision="fp64fast"):
-    """2D relativistic Kelvin-Helmholtz, n^2 periodic, WENOZ + HLLC + SSP-RK3, CFL 0.4."""
+def run_kh(n=2048, steps=2000, warmup=3, precision="fp64fast", fp32_error=True):
+    """2D relativistic Kelvin-Helmholtz, n^2 periodic, WENOZ + HLLC + SSP-RK3, CFL 0.4,
+    for the configured 2000 steps: the rate over all of them (after `warmup`
+    untimed steps from the same initial state), unrecovered cells, and the FP32
+    form's relative L1 error per variable against the exact FP64 form after
+    the same number of steps."""
     from verona_amd import DeviceGrid
     from verona_amd.problems import kelvin_helmholtz
     dx = 1.0 / n
     bc = ["periodic"] * 4 + ["none", "none"]
-    with DeviceGrid(2, n, n, 1, GAMMA) as dev:
-        dev.upload("P", kelvin_helmholtz(n, gamma=GAMMA))
-        dev.fill_ghosts(bc)
-        dev.prim_to_cons()
+    P0 = kelvin_helmholtz(n, gamma=GAMMA)
+
+    def run(prec, nsteps, warm, timed):
+        with DeviceGrid(2, n, n, 1, GAMMA) as dev:
+            dev.upload("P", P0)
+            dev.fill_ghosts(bc)
+            dev.prim_to_cons()
+            dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision=prec,
+                         n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, bc=bc)
+            for _ in range(warm):
+                dev.step()
+            if warm:
+                # the timed run starts from the initial state again
+                dev.upload("P", P0)
+                dev.fill_ghosts(bc)
+                dev.prim_to_cons()
+            dev.sync()
+            t0 = time.perf_counter()
+            t = 0.0
+            for _ in range(nsteps):
+                if timed:
+                    dev.step()
+                else:
+                    t += dev.step(want_dt=True)
+            dev.sync()
+            el = time.perf_counter() - t0
+            nbad = dev.counters()[0]
+            Pn = None if timed else dev.download("P")[:, 3:-3, 3:-3]
+        return el, nbad, Pn, t
+
+    el, nbad, _, _ = run(precision, steps, warmup, True)
+    out = {"what": f"2D Kelvin-Helmholtz {n}^2 periodic, WENOZ+HLLC+SSP-RK3, CFL 0.4, {precision}, "
+                   f"all {steps} configured steps",
+           "value": n * n * steps / el, "unit": "zone-updates/s", "steps": steps,
+           "ms_per_step": el / steps * 1e3, "n_unrecovered": nbad}
+    if fp32_error:
+        _, nb64, A, t64 = run("fp64", steps, 0, False)
+        _, nb32, B, t32 = run("fp32", steps, 0, False)
+        out["fp32_error_at_t_end"] = {
+            "what": f"FP32 recon/Riemann vs exact FP64 after {steps} steps (each form's own CFL dt)",
+            "t_end_fp64": t64, "t_end_fp32": t32, "n_unrecovered": [nb64, nb32],
+            "rel_l1_per_var": dict(zip(["rho", "u", "u^x", "u^y"],
+                                       [float(np.abs(B[v] - A[v]).sum() / np.abs(A[v]).sum()) for v in range(4)]))}
+    return out
+
+
+def run_jet_developed(n=384, advance=700, steps=20, precision="fp64fast", log=None):
+    """The jet after `advance` untimed steps (the beam, its bow shock and
+    cocoon fill a third of the grid: shocked cells at Gamma up to 10), then
+    `steps` timed steps; the early window (steps 6-25 from rest) is the
+    headline's.  The method (WENO-Z + HLLC + SSP-RK3, CFL 0.4, no positivity
+    limiter, as Verona's) produces an unrecoverable state on the jet axis
+    after ~830 steps at 384^3, so the window stays before that."""
+    say = log or (lambda m: None)
+    from verona_amd import DeviceGrid
+    dx = 1.0 / n
+    nz = nozzle(n)
+    phis = phi_stream(advance + steps)
+    with DeviceGrid(3, n, n, n, GAMMA) as dev:
+        dev.upload("P", jet_state(n))
+        dev.set_nozzle(nz["radius"], dx, dx, nz["rho"], nz["u"], nz["umag"], nz["eps"])
+        dev.fill_ghosts(BC_JET)
         dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision=precision,
-                     n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, bc=bc)
-        for _ in range(warmup):
-            dev.step()
+                     n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx, bc=BC_JET)
+        dev.prim_to_cons()
+        t = 0.0
+        for s in range(advance):
+            t += dev.step(phi=phis[s], want_dt=True)
+            if (s + 1) % 100 == 0:
+                say(f"jet developed: step {s + 1}, t {t:.4f}")
         dev.sync()
+        resc0 = dev.counters()[1]
+        dev.timing(True)
         t0 = time.perf_counter()
-        for _ in range(steps):
-            dev.step()
+        for s in range(steps):
+            dev.step(phi=phis[advance + s])
         dev.sync()
         el = time.perf_counter() - t0
-        nbad = dev.counters()[0]
-    return {"what": f"2D Kelvin-Helmholtz {n}^2 periodic, WENOZ+HLLC+SSP-RK3, CFL 0.4, {precision}",
-            "value": n * n * steps / el, "unit": "zone-updates/s", "steps": steps,
-            "ms_per_step": el / steps * 1e3, "n_unrecovered": nbad}
+        rep = dev.timing_report()
+        dev.timing(False)
+        nbad, nresc = dev.counters()
+        Pz = dev.download("P")[4, 3:-3, 3:-3, 3:-3]
+    lor = np.sqrt(1.0 + Pz * Pz)  # longitudinal part of the Lorentz factor
+    head = np.flatnonzero((lor > 1.5).any(axis=(0, 1)))
+    return {"what": f"jet {n}^3, {precision}, {steps} timed steps after {advance} untimed steps",
+            "value": n ** 3 * steps / el, "unit": "zone-updates/s", "ms_per_step": el / steps * 1e3,
+            "t_at_window": t, "jet_head_z_cells": int(head.max()) + 1 if head.size else 0,
+            "n_unrecovered": nbad, "n_rescued_in_window": nresc - resc0, "n_rescued_total": nresc,
+            "kernels_ms": {k: {"launches": c, "mean_ms": m} for k, (c, m) in rep.items()}}
 
 
 def _tube_subset(sel, lengths, rho, u, ux, dt, dx):
@@ -236,7 +311,7 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(budget_s=12.0, budget_ref_s=6.0, n=128):
+def cpu_baseline(budget_s=12.0, budget_ref_s=6.0, n=128, all_configs=True):
     """CPU oracle (plain-C restatement of Verona's algorithm, OpenMP) on a
     bounded sample of the same workload: an n^3 grid, steps until about
     budget_s seconds of wall time have elapsed -- the benchmark's method
@@ -270,16 +345,119 @@ def cpu_baseline(budget_s=12.0, budget_ref_s=6.0, n=128):
     ref = O.make_mode(ref=True, rec="wenoz", riemann="hll", integrator="rk2", n_iter=50,
                       tol=1e-10, gamma=GAMMA, floor=1e-7, dt=0.4 / (3.0 / dx), dx=dx, dy=dx, dz=dx)
     steps_r, el_r = timed(ref, budget_ref_s)
+    configs = cpu_baselines_by_config(O, cores) if all_configs else None
     return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": cores, "kind": "port",
             "sample": f"CPU oracle (C restatement of Verona's algorithm, OpenMP) on the same jet "
                       f"workload at {n}^3, {steps} SSP-RK3 WENOZ+HLLC steps, {el:.1f} s",
             "ref_semantics_value": n ** 3 * steps_r / el_r,
             "ref_semantics_sample": f"same, Verona's method (WENOZ+HLL, midpoint RK2, fixed dt), "
                                     f"{steps_r} steps, {el_r:.1f} s",
-            "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "what": "CPU restatement of Verona's algorithm (the oracle), not Verona itself",
+            "configs": configs}
 
 
-def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=6000, reps=5, log=None, quick=False):
+def run_c1_gpu():
+    """The 1D relativistic shock tube on the GPU (Verona1D.HARM_HLL over one
+    tube): N = 1024, rho 10 / 1, p 13.33 / 1e-6, minmod + HLLC, RK2,
+    dt = 0.4 dx, outflow (frozen) edges, to t = 0.4 (1025 steps)."""
+    from verona_amd.problems import shock_tube
+    from verona_amd.tubes import Tubes
+    n = 1024
+    rho, u, ux = shock_tube(n)
+    dx = 1.0 / n
+    steps, t = 0, 0.0
+    while t < 0.4:
+        t += 0.4 * dx
+        steps += 1
+    with Tubes([n]) as tb:
+        tb.upload(rho, u, ux, [0.4 * dx], [dx])
+        tb.set_method("minmod", "hllc", 50, 1e-10)
+        tb.run(steps, 128, 1)  # graph capture + warm-up
+        tb.upload(rho, u, ux, [0.4 * dx], [dx])
+        ms = tb.run(steps, 128, 1)
+    return {"what": "1D shock tube N=1024, minmod+HLLC, RK2, to t=0.4, one tube (one workgroup)",
+            "steps": steps, "ms": ms, "value": n * steps / (ms * 1e-3), "unit": "cell-steps/s"}
+
+
+def cpu_baselines_by_config(O, cores, budget_s=4.0):
+    """The CPU oracle (C restatement of Verona's algorithm) on the other
+    configurations, each on a bounded sample, all host cores: C1 in full, C2
+    (Kelvin-Helmholtz 2048^2) for the steps that fit the budget, C4 (2^24
+    con2prim cells) from the generic and the warm start, C5 (the 4,096
+    tubes) for a fixed number of steps with the tubes spread over the cores."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+
+    from verona_amd.c2p import c4_states, warm_guess
+    from verona_amd.problems import kelvin_helmholtz, shock_tube
+    from verona_amd.tubes import shock_tube_batch
+    out = {}
+    print("cpu baseline C1", file=sys.stderr, flush=True)
+    # C1: the whole run (the 1D oracle is Verona1D's loop, threaded inside con2prim)
+    n = 1024
+    rho, u, ux = shock_tube(n)
+    P1 = np.ascontiguousarray(np.stack([rho, u, ux]))
+    t0 = time.perf_counter()
+    steps, _ = O.run1(P1, 0.4 / n, 1.0 / n, 0.4, 5.0 / 3.0, "minmod", "hllc", 50, 1e-10)
+    el = time.perf_counter() - t0
+    out["C1_shock_tube"] = {"value": n * steps / el, "unit": "cell-steps/s", "seconds": el,
+                            "sample": f"the whole run: 1024 cells, minmod+HLLC, RK2, {steps} steps to t=0.4"}
+    print("cpu baseline C2", file=sys.stderr, flush=True)
+    # C2: Kelvin-Helmholtz 2048^2, the benchmark method, steps within the budget
+    n = 2048
+    P2 = np.ascontiguousarray(kelvin_helmholtz(n, gamma=GAMMA).T)
+    m2 = O.make_mode(ref=False, rec="wenoz", riemann="hllc", integrator="rk3", n_iter=50, tol=1e-10,
+                     gamma=GAMMA, floor=1e-7, cfl=0.4, dx=1.0 / n, dy=1.0 / n)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        O.run2(P2, m2, [1, 1, 1, 1], 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            break
+    out["C2_kelvin_helmholtz"] = {"value": n * n * steps / el, "unit": "zone-updates/s", "seconds": el,
+                                  "sample": f"2048^2 periodic, WENOZ+HLLC+SSP-RK3, CFL 0.4, {steps} steps "
+                                            "from the initial state"}
+    del P2
+    print("cpu baseline C4", file=sys.stderr, flush=True)
+    # C4: 2^24 held-out cells, the 1DW Newton from the generic and the warm start
+    n = 1 << 24
+    U, P = c4_states(n, seed=42, gamma=GAMMA)
+    res = {}
+    for name, Pg in (("generic", np.tile([1.0, 1.0, 0.0, 0.0, 0.0], (n, 1))), ("warm", warm_guess(P))):
+        W0 = O.c2p4_w0_from_prims(Pg, GAMMA)
+        t0 = time.perf_counter()
+        _, it, cv = O.c2p4_solve(U, W0, GAMMA, 100, 1e-12)
+        el = time.perf_counter() - t0
+        res[name] = {"value": n / el, "unit": "cells/s", "seconds": el, "iters_mean": float(it.mean()),
+                     "not_converged": int((~cv).sum())}
+    del U, P
+    out["C4_con2prim"] = {**res, "sample": "2^24 cells of the benchmark law (seed 42), 1DW Newton to "
+                                           "|dW| < 1e-12 W, solve only"}
+    print("cpu baseline C5", file=sys.stderr, flush=True)
+    # C5: every tube, a fixed number of steps, tubes spread over the cores
+    # (one OpenMP thread per tube run)
+    lengths, rho, u, ux, dt, dx = shock_tube_batch(4096, 256, 8192, seed=42)
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    nst = 4
+    gomp = ctypes.CDLL("libgomp.so.1")
+
+    def one(t):
+        gomp.omp_set_num_threads(1)
+        Pt = np.ascontiguousarray(np.stack([rho[off[t]:off[t + 1]], u[off[t]:off[t + 1]], ux[off[t]:off[t + 1]]]))
+        O.run1(Pt, dt[t], dx[t], 1e30, 5.0 / 3.0, "ppm", "hllc", 50, 1e-10, max_steps=nst)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(one, np.argsort(-lengths)))
+    el = time.perf_counter() - t0
+    out["C5_tubes"] = {"value": int(lengths.sum()) * nst / el, "unit": "cell-steps/s", "seconds": el,
+                       "sample": f"all 4096 tubes ({int(lengths.sum())} cells), PPM+HLLC, RK2, {nst} steps"}
+    return out
+
+
+def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=20000, reps=5, log=None, quick=False):
     """Learned-guess con2prim vs plain Newton on held-out cells of the benchmark
     law (seed 42): every guess finished by the same FP64 1DW Newton to
     |dW| < 1e-12 W.  Training pairs: the solver's own dumps (2D Kelvin-Helmholtz,
@@ -451,6 +629,7 @@ def main():
     ap.add_argument("--tubes-only", action="store_true", help="only the batched 1D tubes leg")
     ap.add_argument("--kh-only", action="store_true", help="only the 2D Kelvin-Helmholtz 2048^2 leg")
     ap.add_argument("--tubes-steps", type=int, default=500, help="steps of the batched tubes leg")
+    ap.add_argument("--kh-steps", type=int, default=40, help="steps of the --kh-only leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of one 384^3 slot (known bytes) for counter calibration")
     ap.add_argument("--precision", default="fp64fast", choices=["fp64", "fp64fast"],
@@ -479,7 +658,7 @@ def main():
         print(json.dumps({"batched_tubes": run_tubes_both(steps=args.tubes_steps)}), flush=True)
         return
     if args.kh_only:
-        print(json.dumps({"kelvin_helmholtz": run_kh()}), flush=True)
+        print(json.dumps({"kelvin_helmholtz": run_kh(steps=args.kh_steps, fp32_error=False)}), flush=True)
         return
     cells = n ** 3
     el, rep, nbad, nresc = run_gpu(n, K, W, ref=False, precision=args.precision)
@@ -563,6 +742,10 @@ def main():
         out["fp32_form"]["error_vs_fp64"] = fp32_error(n, 10)
         log("kelvin-helmholtz")
         out["kelvin_helmholtz"] = run_kh()
+        log("jet, developed flow")
+        out["jet_developed"] = run_jet_developed(n, precision=args.precision, log=log)
+        log("shock tube C1")
+        out["shock_tube_c1"] = run_c1_gpu()
         log("batched tubes")
         out["batched_tubes"] = run_tubes_both()
     if not args.no_c2p:

@@ -13,7 +13,7 @@
  *   VRN_GUESS_GENERIC   P = (1, 1, 0, 0, 0) for every cell (no history)
  *   VRN_GUESS_WARM      the caller's previous primitives (UtoP's in-out P)
  *   VRN_GUESS_ANALYTIC  W0 = E = -T^t_t, a lower bound of W
- *   VRN_GUESS_LEARNED   W0 from a 5 -> 64 -> 64 -> 2 MLP evaluated with BF16
+ *   VRN_GUESS_LEARNED   W0 from a 4 -> 16 -> 16 -> 2 MLP evaluated with BF16
  *                       MFMA (FP32 accumulation) inside the solve kernel
  *
  * Every guess is finished by the same FP64 Newton iteration (the reference's
@@ -43,12 +43,12 @@ extern "C" {
 #define VRN_GUESS_ANALYTIC 2
 #define VRN_GUESS_LEARNED 3
 
-/* MLP 5 -> 64 -> 64 -> 2, parameters packed as W1[64][5], b1[64], W2[64][64],
- * b2[64], W3[2][64], b3[2] (row = output unit), FP32. */
-#define VRN_MLP_NIN 5
-#define VRN_MLP_NHID 64
+/* MLP 4 -> 16 -> 16 -> 2, parameters packed as W1[16][4], b1[16], W2[16][16],
+ * b2[16], W3[2][16], b3[2] (row = output unit), FP32. */
+#define VRN_MLP_NIN 4
+#define VRN_MLP_NHID 16
 #define VRN_MLP_NOUT 2
-#define VRN_MLP_NPARAM 4674
+#define VRN_MLP_NPARAM 386
 
 typedef struct vrn_c2p vrn_c2p;
 
@@ -108,23 +108,25 @@ int vrn_c2p_download(vrn_c2p *h, double *P, double *W0, uint8_t *iters);
 int vrn_mlp_set(vrn_c2p *h, const float *params);
 int vrn_mlp_get(vrn_c2p *h, float *params);
 
-/* Features X[n][5] and training targets T[n][2] of (U, P) pairs as the
+/* Features X[n][VRN_MLP_NIN] and training targets T[n][2] of (U, P) pairs as the
  * learned guess defines them; valid[i] = 0 marks pairs outside the model's
  * domain.  P may be NULL (then T is not written). */
 int vrn_mlp_features(vrn_c2p *h, int64_t n, const double *U, const double *P, float *X, float *T,
                      uint8_t *valid);
 
-/* Network outputs Y[n][2] for features X[n][5]: the BF16 MFMA inference
+/* Network outputs Y[n][2] for features X[n][VRN_MLP_NIN]: the BF16 MFMA inference
  * kernel of the solve, on its own. */
 int vrn_mlp_infer(vrn_c2p *h, int64_t n, const float *X, float *Y);
 
-/* Loss (mean over samples of sum_o w_o (y_o - t_o)^2) and its gradient
- * with respect to params, by the training kernels.  qat = 1 evaluates the
- * network the way the inference kernel does (BF16 rounding of the inputs,
- * W1 with b1, W2, W3 and both hidden layers; FP32 b2, b3 and accumulation;
- * straight-through gradients). */
+/* Loss (mean over samples of s (y_0 - t_0)^2 + w_1 (y_1 - t_1)^2) and its
+ * gradient with respect to params, by the training kernels.  qat = 1
+ * evaluates the network the way the inference kernel does (BF16 rounding of
+ * the inputs, W1, W2, W3 and both hidden layers; FP32 b1, b2, b3 and
+ * accumulation; straight-through gradients).  s = 1 + (q / q_ref)^2 with
+ * q = X_2^2 e^{t_0} (= log(W/E) of the pair: the guess's relative W error is
+ * q times the error of y_0) when q_ref > 0, else 1. */
 int vrn_mlp_grad(vrn_c2p *h, int64_t n, const float *X, const float *T, const float *params,
-                 int qat, float *loss, float *grad);
+                 int qat, float q_ref, float *loss, float *grad);
 
 typedef struct vrn_mlp_train_params {
     int steps;        /* Adam steps */
@@ -133,6 +135,7 @@ typedef struct vrn_mlp_train_params {
     float lr_min;
     float beta1, beta2, eps;
     float w_out1;     /* loss weight of the second output (log Gamma) */
+    float q_ref;      /* > 0: weight of the first output's error, as in vrn_mlp_grad */
     int qat;          /* train through the inference kernel's BF16 rounding */
     uint64_t seed;    /* initialisation and minibatch sampling */
 } vrn_mlp_train_params;

@@ -1,16 +1,17 @@
-"""NumPy reference of the learned con2prim guess network (5 -> 64 -> 64 -> 2,
+"""NumPy reference of the learned con2prim guess network (4 -> 16 -> 16 -> 2,
 ReLU), TEST INFRASTRUCTURE ONLY: the checker of the HIP training and
 inference kernels.  Nothing in the product imports it.
 
-Parameter packing (4,674 floats): W1[64][5], b1[64], W2[64][64], b2[64],
-W3[2][64], b3[2], row = output unit.
+Parameter packing (386 floats): W1[16][4], b1[16], W2[16][16], b2[16],
+W3[2][16], b3[2], row = output unit.
 
-Loss: mean over samples of (y0 - t0)^2 + w1 (y1 - t1)^2.
+Loss: mean over samples of s (y0 - t0)^2 + w1 (y1 - t1)^2, s = 1 + (q/q_ref)^2
+with q = X[:, 2]^2 exp(t0) when q_ref > 0, else s = 1.
 
 `qat=True` evaluates the network the way the BF16 inference kernel does:
-inputs, W1, b1 (folded into layer 1 as the weight of a constant input), W2,
-W3 and both hidden layers rounded to BF16 (round to nearest even); b2, b3
-and all sums in full precision.  The backward pass treats the rounding as the
+inputs, W1, W2, W3 and both hidden layers rounded to BF16 (round to nearest
+even); b1, b2, b3 (the kernel's FP32 accumulator initialisation) and all
+sums in full precision.  The backward pass treats the rounding as the
 identity (straight-through), with the rounded values in every product.
 
 How it is pinned: `loss_grad` against central finite differences of
@@ -19,11 +20,11 @@ transposed weight or a wrong ReLU mask fails.
 """
 import numpy as np
 
-NIN, NH, NOUT = 5, 64, 2
+NIN, NH, NOUT = 4, 16, 2
 SHAPES = [("W1", (NH, NIN)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)), ("W3", (NOUT, NH)),
           ("b3", (NOUT,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
-assert NPARAM == 4674
+assert NPARAM == 386
 
 
 def unpack(params):
@@ -56,8 +57,8 @@ def _q(a, qat):
 
 def forward(params, X, qat=False, dtype=np.float64):
     p = unpack(params)
-    W1, W2, W3, b1 = (_q(p[k], qat).astype(dtype) for k in ("W1", "W2", "W3", "b1"))
-    b2, b3 = (p[k].astype(dtype) for k in ("b2", "b3"))
+    W1, W2, W3 = (_q(p[k], qat).astype(dtype) for k in ("W1", "W2", "W3"))
+    b1, b2, b3 = (p[k].astype(dtype) for k in ("b1", "b2", "b3"))
     x = _q(np.asarray(X, dtype=np.float32), qat).astype(dtype)
     z1 = x @ W1.T + b1
     h1 = np.maximum(z1, 0)
@@ -71,12 +72,22 @@ def forward(params, X, qat=False, dtype=np.float64):
     return dict(x=x, z1=z1, h1=h1, z2=z2, h2=h2, y=y, W1=W1, W2=W2, W3=W3)
 
 
-def loss_grad(params, X, T, w1=1.0, qat=False, dtype=np.float64):
+def _weights(X, T, w1, q_ref, dtype):
+    """Per-sample loss weights [n][2]."""
+    n = len(X)
+    s = np.ones(n, dtype=dtype)
+    if q_ref > 0:
+        q = np.asarray(X, dtype=dtype)[:, 2] ** 2 * np.exp(np.asarray(T, dtype=dtype)[:, 0])
+        s = 1.0 + (q / q_ref) ** 2
+    return np.stack([s, np.full(n, w1, dtype=dtype)], axis=1)
+
+
+def loss_grad(params, X, T, w1=1.0, qat=False, dtype=np.float64, q_ref=0.0):
     """(loss, gradient[NPARAM]) of the mean weighted squared error."""
     f = forward(params, X, qat, dtype)
     n = len(X)
     e = f["y"] - np.asarray(T, dtype=dtype)
-    wv = np.array([1.0, w1], dtype=dtype)
+    wv = _weights(X, T, w1, q_ref, dtype)
     loss = float(((e * e) * wv).sum() / n)
     dy = 2.0 * e * wv / n
     gW3 = dy.T @ f["h2"]
@@ -91,10 +102,10 @@ def loss_grad(params, X, T, w1=1.0, qat=False, dtype=np.float64):
     return loss, g
 
 
-def loss_only(params, X, T, w1=1.0, dtype=np.float64):
+def loss_only(params, X, T, w1=1.0, dtype=np.float64, q_ref=0.0):
     f = forward(params, X, False, dtype)
     e = f["y"] - np.asarray(T, dtype=dtype)
-    return float(((e * e) * np.array([1.0, w1], dtype=dtype)).sum() / len(X))
+    return float(((e * e) * _weights(X, T, w1, q_ref, dtype)).sum() / len(X))
 
 
 def he_init(seed):

@@ -96,17 +96,15 @@ void orc4_features(const double *U, long n, float *X, double *E, double *z, unsi
         const double rE = 1.0 / e;
         const double zz = (tau * (e + D) - S2) * (rE * rE);
         const int good = D > 0.0 && tau > 0.0 && zz > 0.0 && zz < 1.0 && isfinite(e) && isfinite(S2);
-        float *x = X + 5 * c;
+        float *x = X + 4 * c;
         if (good) {
-            const float tf = (float)tau;
-            const float Sm = sqrtf((float)S2);
-            x[0] = 0.1f * logf((float)D);
-            x[1] = 0.1f * logf(tf);
-            x[2] = 0.2f * logf(1.0f + Sm / tf);
-            x[3] = Sm * (float)rE;
-            x[4] = sqrtf((float)zz);
+            const float zf = (float)zz;
+            x[0] = (float)(D * rE);
+            x[1] = sqrtf((float)S2) * (float)rE;
+            x[2] = sqrtf(zf);
+            x[3] = 0.1f * logf(zf);
         } else {
-            x[0] = x[1] = x[2] = x[3] = x[4] = 0.0f;
+            x[0] = x[1] = x[2] = x[3] = 0.0f;
         }
         E[c] = e;
         z[c] = zz;

@@ -419,10 +419,10 @@ def c2p4_solve(U, W0, gamma, n_iter, rtol):
 
 
 def c2p4_features(U):
-    """Features X [n][5] (float32), E, z and the domain flag of the learned guess."""
+    """Features X [n][4] (float32), E, z and the domain flag of the learned guess."""
     U = _f64(U)
     n = U.shape[0]
-    X = np.empty((n, 5), dtype=np.float32)
+    X = np.empty((n, 4), dtype=np.float32)
     E = np.empty(n)
     z = np.empty(n)
     ok = np.zeros(n, dtype=np.uint8)

@@ -18,21 +18,29 @@ from verona_amd.c2p import c4_states, p2u
 G = 4.0 / 3.0
 
 
-def test_mlp_reference_gradient_matches_finite_differences():
+@pytest.mark.parametrize("q_ref", [0.0, 0.5])
+def test_mlp_reference_gradient_matches_finite_differences(q_ref):
     rng = np.random.default_rng(0)
     params = mlp_ref.he_init(3).astype(np.float64)
     params[mlp_ref.NPARAM - 2:] = [0.3, -0.2]
-    X = rng.standard_normal((40, 5))
+    X = rng.standard_normal((40, mlp_ref.NIN))
     T = rng.standard_normal((40, 2))
-    loss, g = mlp_ref.loss_grad(params, X, T, w1=0.7)
+    loss, g = mlp_ref.loss_grad(params, X, T, w1=0.7, q_ref=q_ref)
     idx = rng.choice(mlp_ref.NPARAM, 300, replace=False)
     h = 1e-6
+    lo = lambda p: mlp_ref.loss_only(p, X, T, 0.7, q_ref=q_ref)  # noqa: E731
     for k in idx:
         e = np.zeros_like(params)
         e[k] = h
-        fd = (mlp_ref.loss_only(params + e, X, T, 0.7) - mlp_ref.loss_only(params - e, X, T, 0.7)) / (2 * h)
+        fd = (lo(params + e) - lo(params - e)) / (2 * h)
         assert abs(fd - g[k]) <= 1e-6 * (1 + abs(fd)), (k, fd, g[k])
-    assert abs(loss - mlp_ref.loss_only(params, X, T, 0.7)) < 1e-14
+    assert abs(loss - lo(params)) < 1e-14 * max(1.0, loss)
+    if q_ref > 0:
+        # the weighted loss is the plain one with each sample's first error scaled
+        s = 1 + (X[:, 2] ** 2 * np.exp(T[:, 0]) / q_ref) ** 2
+        y = mlp_ref.forward(params, X)["y"]
+        want = ((s * (y[:, 0] - T[:, 0]) ** 2 + 0.7 * (y[:, 1] - T[:, 1]) ** 2)).mean()
+        assert abs(loss - want) <= 1e-12 * want
 
 
 def test_bf16_round_ties_to_even():
@@ -112,12 +120,19 @@ def test_features_invariance_scale_and_closed_form(oracle):
     Uf[:, 3] *= -1.0
     Xf, _, zf, okf = oracle.c2p4_features(Uf)
     assert (Xf == X).all() and (zf == z).all() and (okf == ok).all()
-    # scaling (D, tau, S) by k: dimensionless features fixed, logs shift by log(k)/10
+    # scaling (D, tau, S) by k leaves every (dimensionless) feature fixed
     k = 8.0
     Xs, _, zs, _ = oracle.c2p4_features(U * k)
     m = ok
-    assert np.allclose(Xs[m, 3:], X[m, 3:], rtol=1e-6, atol=1e-7)
-    assert np.allclose(Xs[m, :2] - X[m, :2], np.log(k) / 10, atol=1e-5)
+    assert np.allclose(Xs[m], X[m], rtol=1e-6, atol=1e-7)
+    # the features from the primitives: D/E = rho Gamma / E, |S|/E = w Gamma |u| / E,
+    # sqrt(z) and ln(z)/10
+    lorp = np.sqrt(1 + (P[:, 2:] ** 2).sum(1))
+    wp = P[:, 0] + G * P[:, 1]
+    assert np.allclose(X[m, 0], (P[:, 0] * lorp / E)[m], rtol=1e-6)
+    assert np.allclose(X[m, 1], (wp * lorp * np.sqrt((P[:, 2:] ** 2).sum(1)) / E)[m], rtol=1e-6, atol=1e-7)
+    assert np.allclose(X[m, 2] ** 2, z[m], rtol=1e-6)
+    assert np.allclose(X[m, 3], (np.log(z) / 10)[m], rtol=1e-6, atol=1e-6)
     # closed form from the primitives: E^2 - S^2 - D^2 = Gamma^2 (w^2 - rho^2 - 2 w p) + p^2,
     # whose first-order term in p is 2 rho p Gamma^2 / (gamma - 1)
     rho, p = P[:, 0], (G - 1) * P[:, 1]

@@ -62,11 +62,12 @@ def test_features_match_oracle(cp, oracle):
 def test_mfma_inference_exact_on_exact_data(cp):
     progress("mfma layout")
     rng = np.random.default_rng(1)
-    d = {"W1": rng.choice([-1.0, -0.5, 0.0, 0.5, 1.0], (64, 5)), "b1": rng.integers(-4, 5, 64) * 0.25,
-         "W2": rng.choice([-0.5, 0.0, 0.5], (64, 64)), "b2": rng.integers(-4, 5, 64) * 0.25,
-         "W3": rng.choice([-0.25, 0.0, 0.25], (2, 64)), "b3": np.array([0.5, -0.25])}
+    nin, nh = mlp_ref.NIN, mlp_ref.NH
+    d = {"W1": rng.choice([-1.0, -0.5, 0.0, 0.5, 1.0], (nh, nin)), "b1": rng.integers(-4, 5, nh) * 0.25,
+         "W2": rng.choice([-0.5, 0.0, 0.5], (nh, nh)), "b2": rng.integers(-4, 5, nh) * 0.25,
+         "W3": rng.choice([-0.25, 0.0, 0.25], (2, nh)), "b3": np.array([0.5, -0.25])}
     params = mlp_ref.pack(d)
-    X = (rng.integers(-8, 9, (1000, 5)) * 0.25).astype(np.float32)
+    X = (rng.integers(-8, 9, (1000, nin)) * 0.25).astype(np.float32)
     cp.set_params(params)
     Y = cp.infer(X)
     Yr = mlp_ref.forward(params, X, qat=True)["y"]
@@ -86,15 +87,15 @@ def test_mfma_inference_matches_bf16_model(cp):
     assert err.max() < 2e-2 * (1 + np.abs(Yr).max()) and err.mean() < 2e-3 * (1 + np.abs(Yr).mean())
 
 
-@pytest.mark.parametrize("qat", [False, True])
-def test_training_gradient_matches_reference(cp, qat):
-    progress(f"grad qat={qat}")
+@pytest.mark.parametrize("qat,q_ref", [(False, 0.0), (True, 0.0), (True, 0.01)])
+def test_training_gradient_matches_reference(cp, qat, q_ref):
+    progress(f"grad qat={qat} q_ref={q_ref}")
     U, P = c4_states(4096, seed=5, gamma=G)
     X, T, ok = cp.features(U, P)
     X, T = X[ok], T[ok]
     params = mlp_ref.he_init(12)
-    loss, g = cp.grad(X, T, params, qat=qat)
-    lr, gr = mlp_ref.loss_grad(params, X, T, qat=qat)
+    loss, g = cp.grad(X, T, params, qat=qat, q_ref=q_ref)
+    lr, gr = mlp_ref.loss_grad(params, X, T, qat=qat, q_ref=q_ref)
     assert abs(loss - lr) <= 1e-5 * lr
     tol = 1e-3 if qat else 1e-4
     assert np.linalg.norm(g - gr) <= tol * np.linalg.norm(gr)

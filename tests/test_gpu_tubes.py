@@ -129,3 +129,39 @@ def test_graph_replay_and_autotune_are_transparent(oracle):
             res.append(np.stack(tb.download()))
     assert np.array_equal(res[0], res[1], equal_nan=True)
     assert np.array_equal(res[0], res[2], equal_nan=True)
+
+
+def test_full_batch_matches_oracle_fixture():
+    """The whole batched-tubes workload (4,096 tubes, lengths U{256..8192},
+    seed 42, PPM + HLLC, RK2, 500 steps) in the exact FP64 form: the cells
+    Verona1D's floor-free Newton leaves unrecovered (NaN markers) are exactly
+    the oracle's, tube by tube and variable by variable, and every tube's
+    sum of |rho|, |u|, |u^x| over its other cells equals the oracle's
+    (tests/golden/tubes_c5.npz, written by tests/golden/make_tubes_c5.py)."""
+    import os
+
+    from verona_amd.tubes import Tubes, shock_tube_batch
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "tubes_c5.npz"))
+    ntubes, steps = len(fx["lengths"]), int(fx["steps"])
+    progress(f"test_full_batch_matches_oracle_fixture {ntubes} tubes x {steps} steps")
+    lengths, rho, u, ux, dt, dx = shock_tube_batch(ntubes, 256, 8192, seed=int(fx["seed"]))
+    assert np.array_equal(lengths, fx["lengths"])
+    with Tubes(lengths) as tb:
+        tb.upload(rho, u, ux, dt, dx)
+        tb.set_method("ppm", "hllc", 50, 1e-10)
+        tb.run(steps, 256, 4 * tb.cu_count)
+        G = np.stack(tb.download())
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    nan_cells = np.zeros((ntubes, 3), np.int64)
+    abs_sum = np.zeros((ntubes, 3))
+    for t in range(ntubes):
+        P = G[:, off[t]:off[t + 1]]
+        nan = ~np.isfinite(P)
+        nan_cells[t], abs_sum[t] = nan.sum(axis=1), np.where(nan, 0.0, np.abs(P)).sum(axis=1)
+    nan_tubes = np.flatnonzero(nan_cells.sum(axis=1) > 0)
+    rel = np.abs(abs_sum - fx["abs_sum"]) / np.maximum(np.abs(fx["abs_sum"]), 1e-300)
+    progress(f"  NaN tubes: GPU {len(nan_tubes)}, oracle {len(fx['nan_tubes'])}; checksums equal "
+             f"{int((abs_sum == fx['abs_sum']).all(axis=1).sum())}/{ntubes}, max rel diff {rel.max():.2e}")
+    assert np.array_equal(nan_tubes, fx["nan_tubes"])
+    assert np.array_equal(nan_cells, fx["nan_cells"])
+    assert np.array_equal(abs_sum, fx["abs_sum"])

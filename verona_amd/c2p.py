@@ -22,7 +22,7 @@ from ._lib import VeronaHIPError
 
 GUESS = {"generic": 0, "warm": 1, "analytic": 2, "learned": 3}
 RECORD = 0x100
-NPARAM = 4674
+NIN, NPARAM = 4, 386
 
 _P = C.c_void_p
 _I = C.c_int
@@ -42,7 +42,7 @@ class C2PStats(C.Structure):
 class MLPTrainParams(C.Structure):
     _fields_ = [("steps", _I), ("batch", _I), ("lr", C.c_float), ("lr_min", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("w_out1", C.c_float),
-                ("qat", _I), ("seed", C.c_uint64)]
+                ("q_ref", C.c_float), ("qat", _I), ("seed", C.c_uint64)]
 
 
 C2P_SIGNATURES = {
@@ -57,7 +57,7 @@ C2P_SIGNATURES = {
     "vrn_mlp_get": (_I, [_P, _FP]),
     "vrn_mlp_features": (_I, [_P, _I64, _DP, _DP, _FP, _FP, _U8P]),
     "vrn_mlp_infer": (_I, [_P, _I64, _FP, _FP]),
-    "vrn_mlp_grad": (_I, [_P, _I64, _FP, _FP, _FP, _I, _FP, _FP]),
+    "vrn_mlp_grad": (_I, [_P, _I64, _FP, _FP, _FP, _I, C.c_float, _FP, _FP]),
     "vrn_mlp_train": (_I, [_P, _I64, _DP, _DP, _FP, C.POINTER(MLPTrainParams), _FP, _DP]),
 }
 _lib.SIGNATURES.update(C2P_SIGNATURES)
@@ -174,7 +174,7 @@ class Con2PrimBatch:
     def features(self, U, P=None):
         U, pu = _d(U)
         n = U.shape[0]
-        X = np.empty((n, 5), dtype=np.float32)
+        X = np.empty((n, NIN), dtype=np.float32)
         T = np.empty((n, 2), dtype=np.float32) if P is not None else None
         ok = np.empty(n, dtype=np.uint8)
         pp = None
@@ -191,21 +191,21 @@ class Con2PrimBatch:
         _check(self.lib.vrn_mlp_infer(self.h, X.shape[0], px, Y.ctypes.data_as(_FP)), "vrn_mlp_infer")
         return Y
 
-    def grad(self, X, T, params, qat=False):
+    def grad(self, X, T, params, qat=False, q_ref=0.0):
         X, px = _f(X)
         T, pt = _f(T)
         params, pp = _f(params)
         g = np.empty(NPARAM, dtype=np.float32)
         loss = C.c_float(0)
-        _check(self.lib.vrn_mlp_grad(self.h, X.shape[0], px, pt, pp, int(bool(qat)), C.byref(loss),
+        _check(self.lib.vrn_mlp_grad(self.h, X.shape[0], px, pt, pp, int(bool(qat)), float(q_ref), C.byref(loss),
                                      g.ctypes.data_as(_FP)), "vrn_mlp_grad")
         return float(loss.value), g
 
     def train(self, U, P, init=None, steps=6000, batch=8192, lr=3e-3, lr_min=1e-5, beta1=0.9,
-              beta2=0.999, eps=1e-8, w_out1=1.0, qat=True, seed=7):
+              beta2=0.999, eps=1e-8, w_out1=1.0, q_ref=0.01, qat=True, seed=7):
         U, pu = _d(U)
         P, pp = _d(P)
-        tp = MLPTrainParams(int(steps), int(batch), lr, lr_min, beta1, beta2, eps, w_out1, int(bool(qat)),
+        tp = MLPTrainParams(int(steps), int(batch), lr, lr_min, beta1, beta2, eps, w_out1, q_ref, int(bool(qat)),
                             int(seed))
         curve = np.empty(int(steps), dtype=np.float32)
         ms = C.c_double(0)

@@ -14,18 +14,18 @@
 // state space, and the guess W0 = E (1 + expm1(z e^y0)) never falls below the
 // lower bound E, so the relative guess error is (error of y0) * q -- small
 // exactly where the reference's W_min clamp makes Newton slow (Gamma >> 1).
-// The second output is log Gamma.  Inputs: log D, log tau (tau = E - D),
-// log(1 + |S|/tau), |S|/E and sqrt(z); none depends on the direction of S.
+// The second output is log Gamma.  Inputs: D/E, |S|/E, sqrt(z) and ln(z)/10;
+// none depends on the direction of S.
 //
 // Inference runs inside the solve kernel: a wave takes 64 cells, computes
-// their features, and evaluates all three layers on the matrix cores in the
-// transposed orientation H^T = W X^T (layer 1 with v_mfma_f32_16x16x16_bf16,
-// its bias folded in as a constant input; layer 2 and the output layer with
-// gfx950's v_mfma_f32_16x16x32_bf16), so the accumulator tiles of one layer
-// (rows = hidden units, columns = cells) are, element for element, the B
-// operand of the next: no LDS round trip between layers.  BF16 values:
-// inputs, W1 with b1, W2, W3 and both hidden layers; FP32: accumulation,
-// b2, b3.
+// their features, and evaluates the 4 -> 16 -> 16 -> 2 network on the matrix
+// cores (v_mfma_f32_16x16x16_bf16) in the transposed orientation H^T = W X^T,
+// one 16-cell column tile at a time: every lane supplies its own cell's
+// inputs, and the accumulator tile of one layer (rows = hidden units,
+// columns = cells) is, element for element, the B operand of the next -- no
+// LDS round trip and no shuffle before the outputs.  BF16 values: inputs,
+// W1, W2, W3 and both hidden layers; FP32: the biases (accumulator
+// initialisation) and accumulation.
 //
 // Training (FP32 VALU): one thread per sample runs forward and backward with
 // the parameters in LDS and caches per-sample activations and deltas; weight
@@ -80,7 +80,9 @@ struct Feat {
     bool ok;
 };
 
-// Features of one cell of conserved variables (D, T^t_t, S^2).
+// Features of one cell of conserved variables (D, T^t_t, S^2): D/E, |S|/E,
+// sqrt(z) and ln(z)/10 -- W/E depends on (D/E, |S|/E) alone, and z is the
+// distance from the p = 0 boundary where W/E -> 1.
 __device__ __forceinline__ Feat c4_features(double D, double Qt, double S2)
 {
     Feat f;
@@ -97,16 +99,14 @@ __device__ __forceinline__ Feat c4_features(double D, double Qt, double S2)
     f.ok = D > 0.0 && tau > 0.0 && f.z > 0.0 && f.z < 1.0 && __builtin_isfinite(f.E) &&
            __builtin_isfinite(S2);
     if (f.ok) {
-        // single hardware instructions (v_log_f32 = log2, v_sqrt_f32,
-        // v_rcp_f32; ~1 ulp): the features only steer the initial guess
+        // single hardware instructions (v_sqrt_f32, v_log_f32 = log2; ~1 ulp):
+        // the features only steer the initial guess
         const float LN2 = 0.693147181f;
-        const float tf = (float)tau;
-        const float Sm = __builtin_amdgcn_sqrtf((float)S2);
-        f.x[0] = (0.1f * LN2) * __builtin_amdgcn_logf((float)D);
-        f.x[1] = (0.1f * LN2) * __builtin_amdgcn_logf(tf);
-        f.x[2] = (0.2f * LN2) * __builtin_amdgcn_logf(1.0f + Sm * __builtin_amdgcn_rcpf(tf));
-        f.x[3] = Sm * (float)rE;
-        f.x[4] = __builtin_amdgcn_sqrtf((float)f.z);
+        const float zf = (float)f.z;
+        f.x[0] = (float)(D * rE);
+        f.x[1] = __builtin_amdgcn_sqrtf((float)S2) * (float)rE;
+        f.x[2] = __builtin_amdgcn_sqrtf(zf);
+        f.x[3] = (0.1f * LN2) * __builtin_amdgcn_logf(zf);
     } else {
 #pragma unroll
         for (int k = 0; k < NIN; k++) f.x[k] = 0.0f;
@@ -286,8 +286,6 @@ struct SolveArgs {
 };
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
 // Two floats -> two packed BF16, round to nearest even (compiler conversion:
 // v_cvt_pk_bf16_f32 on gfx950, with the hazard wait states the compiler
@@ -315,34 +313,29 @@ __device__ __forceinline__ unsigned long long relu_pack4(const f32x4 &v)
     const unsigned hi = relu_pk_bf16(cvt_pk_bf16(v[2], v[3]));
     return ((unsigned long long)hi << 32) | lo;
 }
-__device__ __forceinline__ bf16x8 cat8(unsigned long long lo, unsigned long long hi)
-{
-    const u64x2 v = {lo, hi};
-    return __builtin_bit_cast(bf16x8, v);
-}
 
 // Lane l of v_mfma_f32_16x16x16_bf16 supplies A[m = l%16][k = 4 (l/16) + j]
-// and B[k = 4 (l/16) + j][n = l%16] (j = 0..3); v_mfma_f32_16x16x32_bf16
-// the same with k = 8 (l/16) + j (j = 0..7); both leave D[m = 4 (l/16) + j]
-// [n = l%16] (j = 0..3) in lane l.  The hidden layers are evaluated as
-// H^T = W X^T (rows = hidden units, columns = cells).  For a 16x16x32 layer
-// the 32 hidden units of K block kb are ordered so that lane group g holds
-// units 32 kb + 4 g + (0..3) and 32 kb + 16 + 4 g + (0..3): exactly the
-// values lane group g received in the two 16-row accumulator tiles 2 kb and
-// 2 kb + 1 of the previous layer, so a tile pair becomes the next B operand
-// without any data movement; the A fragments (weights) use the same order.
-__device__ __forceinline__ int hidden_of(int kb, int g, int jj)
-{
-    return 32 * kb + (jj < 4 ? 4 * g + jj : 16 + 4 * g + (jj - 4));
-}
-
-// Weights of the learned guess as one wave holds them (registers).
+// and B[k = 4 (l/16) + j][n = l%16] (j = 0..3) and receives D[m = 4 (l/16) +
+// j][n = l%16].  The layers are evaluated as H^T = W X^T (rows = the 16
+// hidden units, columns = 16 cells), one column tile t per 16 cells of the
+// wave (cells 16 t + n = lanes of group t), so:
+//  * layer 1: lane l supplies its own cell's four inputs as B rows
+//    4 (l/16) + j.  For tile t only lane group t carries the tile's cells, so
+//    the A operand of tile t holds W1 in k-block t and zeros elsewhere; the
+//    other groups' inputs meet zero weights.  The bias b1 is the FP32
+//    accumulator initialisation (rows 4 (l/16) + j in lane l).
+//  * layers 2 and 3: an accumulator tile (rows 4 (l/16) + j of column l%16)
+//    is, after ReLU and BF16 packing, exactly the B operand rows the lane
+//    supplies next: no data movement between layers.
+//  * outputs: rows 0 and 1 of the last tile land in lanes 0..15 (cells
+//    16 t + lane), one shuffle per tile delivers them to their owners.
+// 12 MFMA per 64 cells, all operands in 12 VGPRs of weights and 12 of biases.
 struct NetRegs {
-    bf16x4 w1[4];      // layer 1, K = 16: inputs x0..x4, the constant 1 (bias b1), zeros
-    bf16x8 w2[4][2];   // layer 2 [row tile][K block]
-    bf16x8 w3[2];      // output layer, rows 0 and 1 = the two outputs [K block]
-    f32x4 b2[4];       // layer 2 bias, accumulator initialisation
-    f32x4 b3;          // output bias (rows 0, 1 live in lane group 0)
+    bf16x4 w1[4];  // layer 1 A operand of column tile t: W1 in k-block t only
+    bf16x4 w2;     // layer 2 A operand
+    bf16x4 w3;     // output layer A operand, rows 0 and 1
+    f32x4 b1, b2;  // accumulator initialisation (biases)
+    f32x4 b3;      // output bias (rows 0, 1 live in lane group 0)
 };
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
@@ -355,46 +348,25 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
     return r;
 }
 
-__device__ __forceinline__ bf16x8 frag8(const unsigned short *h)
-{
-    typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-    u16x8 v;
-#pragma unroll
-    for (int j = 0; j < 8; j++) v[j] = h[j];
-    return __builtin_bit_cast(bf16x8, v);
-}
-
 __device__ __forceinline__ void load_net(const float *prm, NetRegs &nr, int lane)
 {
     const int m = lane & 15, g = lane >> 4, kq = 4 * g;
+    static_assert(NIN == 4 && NH == 16 && NOUT <= 4, "layout of the 16-wide network");
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        float v[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = kq + j;
-            v[j] = k < NIN ? prm[OFF_W1 + (16 * r + m) * NIN + k] : (k == NIN ? prm[OFF_B1 + 16 * r + m] : 0.0f);
-        }
-        nr.w1[r] = pack4(v[0], v[1], v[2], v[3]);
-#pragma unroll
-        for (int kb = 0; kb < 2; kb++) {
-            unsigned short h[8];
-#pragma unroll
-            for (int jj = 0; jj < 8; jj++) h[jj] = bf16_bits(prm[OFF_W2 + (16 * r + m) * NH + hidden_of(kb, g, jj)]);
-            nr.w2[r][kb] = frag8(h);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) nr.b2[r][j] = prm[OFF_B2 + 16 * r + kq + j];
+    for (int t = 0; t < 4; t++) {
+        const float *w = prm + OFF_W1 + m * NIN;
+        nr.w1[t] = g == t ? pack4(w[0], w[1], w[2], w[3]) : pack4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    const float *w2 = prm + OFF_W2 + m * NH + kq;
+    nr.w2 = pack4(w2[0], w2[1], w2[2], w2[3]);
+    const float *w3 = prm + OFF_W3 + m * NH + kq;
+    nr.w3 = m < NOUT ? pack4(w3[0], w3[1], w3[2], w3[3]) : pack4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-    for (int kb = 0; kb < 2; kb++) {
-        unsigned short h[8];
-#pragma unroll
-        for (int jj = 0; jj < 8; jj++) h[jj] = m < NOUT ? bf16_bits(prm[OFF_W3 + m * NH + hidden_of(kb, g, jj)]) : 0;
-        nr.w3[kb] = frag8(h);
+    for (int j = 0; j < 4; j++) {
+        nr.b1[j] = prm[OFF_B1 + kq + j];
+        nr.b2[j] = prm[OFF_B2 + kq + j];
+        nr.b3[j] = (g == 0 && j < NOUT) ? prm[OFF_B3 + j] : 0.0f;
     }
-#pragma unroll
-    for (int j = 0; j < 4; j++) nr.b3[j] = (g == 0 && j < NOUT) ? prm[OFF_B3 + j] : 0.0f;
 }
 
 // The fragments are formed once per parameter set (one wave) and read by
@@ -413,33 +385,15 @@ template <bool BOTH>
 __device__ __forceinline__ void net_eval(const NetRegs &nr, const float *x, int lane, float &y0, float &y1)
 {
     const int g = lane >> 4;
-    const unsigned p01 = cvt_pk_bf16(x[0], x[1]);
-    const unsigned p23 = cvt_pk_bf16(x[2], x[3]);
-    const unsigned p45 = cvt_pk_bf16(x[4], 1.0f);  // x4 and the constant input of the folded bias
-    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const bf16x4 bx = __builtin_bit_cast(
+        bf16x4, ((unsigned long long)cvt_pk_bf16(x[2], x[3]) << 32) | cvt_pk_bf16(x[0], x[1]));
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const int src = 16 * t + (lane & 15);
-        const unsigned a01 = (unsigned)__shfl((int)p01, src);
-        const unsigned a23 = (unsigned)__shfl((int)p23, src);
-        const unsigned a45 = (unsigned)__shfl((int)p45, src);
-        const unsigned lo = g == 0 ? a01 : (g == 1 ? a45 : 0u);
-        const unsigned hi = g == 0 ? a23 : 0u;
-        const bf16x4 bx = __builtin_bit_cast(bf16x4, ((unsigned long long)hi << 32) | lo);
-        unsigned long long h1[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-            h1[r] = relu_pack4(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w1[r], bx, zero, 0, 0, 0));
-        const bf16x8 b1a = cat8(h1[0], h1[1]), b1b = cat8(h1[2], h1[3]);
-        unsigned long long h2[4];
-#pragma unroll
-        for (int r2 = 0; r2 < 4; r2++) {
-            f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w2[r2][0], b1a, nr.b2[r2], 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w2[r2][1], b1b, acc, 0, 0, 0);
-            h2[r2] = relu_pack4(acc);
-        }
-        f32x4 out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w3[0], cat8(h2[0], h2[1]), nr.b3, 0, 0, 0);
-        out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nr.w3[1], cat8(h2[2], h2[3]), out, 0, 0, 0);
+        const f32x4 z1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w1[t], bx, nr.b1, 0, 0, 0);
+        const bf16x4 h1 = __builtin_bit_cast(bf16x4, relu_pack4(z1));
+        const f32x4 z2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w2, h1, nr.b2, 0, 0, 0);
+        const bf16x4 h2 = __builtin_bit_cast(bf16x4, relu_pack4(z2));
+        const f32x4 out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w3, h2, nr.b3, 0, 0, 0);
         // lanes 0..15 hold the outputs of cells 16 t + lane; deliver to their owners
         const float v0 = __shfl(out[0], lane & 15);
         if (t == g) y0 = v0;
@@ -749,6 +703,7 @@ struct TrainArgs {
     const float *prm;
     float *cache;
     float w1;
+    float q_ref;  // > 0: the first output's squared error weighted by 1 + (q / q_ref)^2
     int qat;
 };
 
@@ -757,8 +712,8 @@ __global__ void __launch_bounds__(256) k_train_fwd_bwd(TrainArgs a)
     __shared__ float w[NPARAM];
     for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) {
         float v = a.prm[p];
-        // BF16 in inference: W1 with b1 (folded into layer 1), W2, W3
-        const bool is_w = p < OFF_B2 || (p >= OFF_W3 && p < OFF_B3);
+        // BF16 in inference: W1, W2, W3 (the biases are FP32 accumulator inputs)
+        const bool is_w = p < OFF_B1 || (p >= OFF_W2 && p < OFF_B2) || (p >= OFF_W3 && p < OFF_B3);
         w[p] = (a.qat && is_w) ? bf16_round(v) : v;
     }
     __syncthreads();
@@ -799,12 +754,21 @@ __global__ void __launch_bounds__(256) k_train_fwd_bwd(TrainArgs a)
         y0 = y0 + w[OFF_W3 + j] * h;
         y1 = y1 + w[OFF_W3 + NH + j] * h;
     }
-    const float e0 = y0 - a.T[2 * id], e1 = y1 - a.T[2 * id + 1];
+    const float t0 = a.T[2 * id];
+    const float e0 = y0 - t0, e1 = y1 - a.T[2 * id + 1];
     const float invB = 1.0f / (float)a.B;
-    const float dy0 = 2.0f * e0 * invB, dy1 = 2.0f * a.w1 * e1 * invB;
+    // q = z e^{t0} = log(W/E): the guess W0 = E e^q misses W by q times the
+    // error of y0, so the hot states (large q) are weighted up
+    float wq = 1.0f;
+    if (a.q_ref > 0.0f) {
+        const float sz = a.X[(long)NIN * id + 2];
+        const float r = sz * sz * expf(t0) / a.q_ref;
+        wq = 1.0f + r * r;
+    }
+    const float dy0 = 2.0f * wq * e0 * invB, dy1 = 2.0f * a.w1 * e1 * invB;
     c[C_DY] = dy0;
     c[C_DY + 1] = dy1;
-    c[C_L] = (e0 * e0 + a.w1 * e1 * e1) * invB;
+    c[C_L] = (wq * e0 * e0 + a.w1 * e1 * e1) * invB;
     // dz2 = (W3^T dy) * [z2 > 0]   (h2 > 0 <=> z2 > 0)
     float dh1[NH];
 #pragma unroll
@@ -982,7 +946,7 @@ int ensure_pairs(vrn_c2p *h, long n)
 
 // One forward/backward + gradient reduction over B samples (cache -> grad).
 int grad_pass(vrn_c2p *h, const float *X, const float *T, const int *idx, long n_idx, int B,
-              unsigned long long seed, int step, const float *prm, float w1, int qat)
+              unsigned long long seed, int step, const float *prm, float w1, float q_ref, int qat)
 {
     TrainArgs ta;
     ta.X = X;
@@ -995,6 +959,7 @@ int grad_pass(vrn_c2p *h, const float *X, const float *T, const int *idx, long n
     ta.prm = prm;
     ta.cache = h->cache;
     ta.w1 = w1;
+    ta.q_ref = q_ref;
     ta.qat = qat;
     hipLaunchKernelGGL(k_train_fwd_bwd, dim3((B + 255) / 256), dim3(256), 0, h->s, ta);
     const int nsplit = B < NSPLIT ? B : NSPLIT;
@@ -1281,7 +1246,7 @@ int vrn_mlp_infer(vrn_c2p *h, int64_t n, const float *X, float *Y)
 }
 
 int vrn_mlp_grad(vrn_c2p *h, int64_t n, const float *X, const float *T, const float *params, int qat,
-                 float *loss, float *grad)
+                 float q_ref, float *loss, float *grad)
 {
     if (!h || !X || !T || !params || !grad) return fail(-1, "vrn_mlp_grad: NULL argument");
     if (n < 1 || n > MAX_BATCH) return fail(-1, "vrn_mlp_grad: n = %lld outside [1, %d]", (long long)n, MAX_BATCH);
@@ -1290,7 +1255,7 @@ int vrn_mlp_grad(vrn_c2p *h, int64_t n, const float *X, const float *T, const fl
     HIPCHK(hipMemcpyAsync(h->X, X, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->T, T, sizeof(float) * 2 * (size_t)n, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->tprm, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, h->s));
-    if (int rc = grad_pass(h, h->X, h->T, nullptr, n, (int)n, 0, 0, h->tprm, 1.0f, qat ? 1 : 0)) return rc;
+    if (int rc = grad_pass(h, h->X, h->T, nullptr, n, (int)n, 0, 0, h->tprm, 1.0f, q_ref, qat ? 1 : 0)) return rc;
     std::vector<float> g(NPARAM + 1);
     HIPCHK(hipMemcpyAsync(g.data(), h->grad, sizeof(float) * (NPARAM + 1), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
@@ -1343,7 +1308,7 @@ int vrn_mlp_train(vrn_c2p *h, int64_t n, const double *U, const double *P, const
         bc1 *= tp->beta1;
         bc2 *= tp->beta2;
         rc = grad_pass(h, h->X, h->T, h->idx, (long)ids.size(), tp->batch, tp->seed, s, h->params, tp->w_out1,
-                       tp->qat);
+                       tp->q_ref, tp->qat);
         if (rc) break;
         hipLaunchKernelGGL(k_adam, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, h->s, h->params, h->m, h->v,
                            (const float *)h->grad, (float)lr, tp->beta1, tp->beta2, tp->eps, (float)(1.0 - bc1),

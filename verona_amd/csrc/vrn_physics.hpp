@@ -136,16 +136,29 @@ template <class T> struct WenoEps;
 template <> struct WenoEps<double> { static constexpr double v = 1.0e-42; };
 template <> struct WenoEps<float> { static constexpr float v = 1.0e-30f; };
 
-// fluxlimiter.jl:19-71  WENOZ(q_{i-2..i+2}) -> lo = q at i-1/2, hi = q at i+1/2
+// WENO-Z curvature term 13/12 (a - 2 b + c)^2 of three consecutive cells,
+// evaluated as (a + c) - 2 b: the smoothness indicators of the stencils
+// centred on cells i-1, i, i+1 (fluxlimiter.jl:45-50) use it with the same
+// three cells in the same order up to the commutative first sum, so one
+// value serves all three positions bit for bit (the marching sweeps compute
+// it once per cell and carry it).
+template <class T> __device__ __forceinline__ T wz_curv(T a, T b, T c)
+{
+    return T(13.0) / T(12.0) * sq(a + c - T(2) * b);
+}
+
+// fluxlimiter.jl:19-71  WENOZ(q_{i-2..i+2}) -> lo = q at i-1/2, hi = q at i+1/2,
+// from the five values and the curvature terms k0 = wz_curv(q_{i-2}, q_{i-1},
+// q_i), k1 = wz_curv(q_{i-1}, q_i, q_{i+1}), k2 = wz_curv(q_i, q_{i+1}, q_{i+2}).
 // SHARED_RCP (fast FP64 form): both faces' normalisations from one
 // reciprocal -- fewer instructions, four more live registers per call.
 template <bool SHARED_RCP = false, class T>
-__device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
+__device__ __forceinline__ void wenoz_k(T qm2, T qm1, T q, T qp1, T qp2, T k0, T k1, T k2, T &lo, T &hi)
 {
-    const T b1c = T(13.0) / T(12.0), b2c = T(0.25);
-    T beta0 = b1c * sq(qm2 + q - T(2) * qm1) + b2c * sq(qm2 + T(3) * q - T(4) * qm1);
-    T beta1 = b1c * sq(qm1 + qp1 - T(2) * q) + b2c * sq(qm1 - qp1);
-    T beta2 = b1c * sq(qp2 + q - T(2) * qp1) + b2c * sq(qp2 + T(3) * q - T(4) * qp1);
+    const T b2c = T(0.25);
+    T beta0 = k0 + b2c * sq(qm2 + T(3) * q - T(4) * qm1);
+    T beta1 = k1 + b2c * sq(qm1 - qp1);
+    T beta2 = k2 + b2c * sq(qp2 + T(3) * q - T(4) * qp1);
     const T eps = WenoEps<T>::v;
     T tau5 = dabs(beta0 - beta2);
     // beta + eps >= 1e-42, a normal double, so the fast reciprocal is safe here
@@ -218,6 +231,13 @@ __device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T 
     } else {
         lo = fdiv(nl, dl);
     }
+}
+
+template <bool SHARED_RCP = false, class T>
+__device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
+{
+    wenoz_k<SHARED_RCP>(qm2, qm1, q, qp1, qp2, wz_curv(qm2, qm1, q), wz_curv(qm1, q, qp1), wz_curv(q, qp1, qp2),
+                        lo, hi);
 }
 
 // fluxlimiter.jl:1-13  MINMOD(q_{i-1}, q_i, q_{i+1})

@@ -277,6 +277,11 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     // lanes per block: 256 in 3D (four o-rows of 64 columns), 64 in 2D
     constexpr int WS = NV == 5 ? 256 : 64;
     __shared__ R win[5 * NV * WS];
+    // WENO-Z: the curvature term of each cell (wz_curv) is computed once and
+    // carried: cell c needs the terms of cells c-1, c and c+1, and the term of
+    // cell x lives in slot x & 1 of a two-slot ring [slot][var][lane].
+    constexpr bool CARRY = REC == REC_WENOZ;
+    __shared__ R curv[CARRY ? 2 * NV * WS : 1];
     const Geo G = a.geo;
     const int tid = threadIdx.x;
     const int x = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
@@ -298,6 +303,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     const double *__restrict__ P = a.P;
     const double *Ui = a.Uacc;
     R *wl = win + tid;  // slot s, variable v at wl[(s * NV + v) * WS]
+    R *kl = curv + tid;  // curvature term of cell x, variable v at kl[((x & 1) * NV + v) * WS]
 
     // ring: cell m lives in slot (m - (m0 - 3)) mod 5
     double nx[NV];
@@ -307,6 +313,13 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
         for (int k = 0; k < 4; k++) wl[(k * NV + v) * WS] = (R)q[(m0 - 3 + k) * sd];
         nx[v] = q[(m0 + 1) * sd];
+        if constexpr (CARRY) {
+            // cells m0-2 and m0-1, the neighbours of the first window's centre
+            kl[(((m0 - 2) & 1) * NV + v) * WS] =
+                wz_curv(wl[(0 * NV + v) * WS], wl[(1 * NV + v) * WS], wl[(2 * NV + v) * WS]);
+            kl[(((m0 - 1) & 1) * NV + v) * WS] =
+                wz_curv(wl[(1 * NV + v) * WS], wl[(2 * NV + v) * WS], wl[(3 * NV + v) * WS]);
+        }
     }
     R upp[NV];
     double Fp[NV];
@@ -338,10 +351,20 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         }
         R lo[NV], up[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++)
+        for (int v = 0; v < NV; v++) {
             // the last sweep (RK combine + con2prim) has no registers to spare
-            recon<REC, !LAST>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
-                       wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
+            if constexpr (CARRY) {
+                const R q2 = wl[(s2 * NV + v) * WS], q3 = wl[(s3 * NV + v) * WS], q4 = wl[(s4 * NV + v) * WS];
+                R *kp = kl + (((c + 1) & 1) * NV + v) * WS;  // cell c-1's term, then cell c+1's
+                const R km = *kp, k0 = kl[((c & 1) * NV + v) * WS], kn = wz_curv(q2, q3, q4);
+                *kp = kn;
+                wenoz_k<!LAST>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], q2, q3, q4, km, k0, kn, lo[v],
+                               up[v]);
+            } else {
+                recon<REC, !LAST>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
+                                  wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
+            }
+        }
         lo[0] = floor_max(fl, lo[0]);
         lo[1] = floor_max(fl, lo[1]);
         up[0] = floor_max(fl, up[0]);
