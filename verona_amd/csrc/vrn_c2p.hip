@@ -96,8 +96,9 @@ __device__ __forceinline__ Feat c4_features(double D, double Qt, double S2)
     // E^2 - S^2 - D^2 = tau (E + D) - S^2, in FP64: it is a difference of
     // nearly equal terms when p << E
     f.z = (tau * (f.E + D) - S2) * (rE * rE);
-    f.ok = D > 0.0 && tau > 0.0 && f.z > 0.0 && f.z < 1.0 && __builtin_isfinite(f.E) &&
-           __builtin_isfinite(S2);
+    // (an infinite or NaN E or S^2 makes z NaN or -inf, so 0 < z < 1 also
+    // rejects non-finite inputs)
+    f.ok = D > 0.0 && tau > 0.0 && f.z > 0.0 && f.z < 1.0;
     if (f.ok) {
         // single hardware instructions (v_sqrt_f32, v_log_f32 = log2; ~1 ulp):
         // the features only steer the initial guess
@@ -133,10 +134,11 @@ __device__ __forceinline__ double learned_w0(const Feat &f, float y0)
     const float LOG2E = 1.44269504f;
     float qh = (float)f.z * __builtin_amdgcn_exp2f(y0 * LOG2E);
     qh = qh < 0.0f ? 0.0f : (qh > 0.5f ? 0.5f : qh);
-    // expm1(qh): Taylor to 4th order below 1/16 (relative error < 2e-7),
-    // exp - 1 above
-    const float em = qh < 0.0625f ? qh * (1.0f + qh * (0.5f + qh * (1.0f / 6.0f + qh * (1.0f / 24.0f))))
-                                  : __builtin_amdgcn_exp2f(qh * LOG2E) - 1.0f;
+    // expm1(qh) on [0, 1/2]: Taylor polynomial to 6th order, relative error
+    // below qh^6 / 5040 (3e-6 at the clamp, 1e-7 at q = 0.29, the largest
+    // log(W/E) of an ideal gas with gamma = 4/3), with no branch
+    const float em =
+        qh * (1.0f + qh * (0.5f + qh * (1.0f / 6.0f + qh * (1.0f / 24.0f + qh * (1.0f / 120.0f + qh * (1.0f / 720.0f))))));
     return f.E + f.E * (double)em;
 }
 
@@ -378,30 +380,41 @@ __global__ void __launch_bounds__(64) k_pack_net(const float *__restrict__ prm, 
     net[threadIdx.x] = nr;
 }
 
+// Lane 16 t + n receives lane n of r_t (t = 0..3, n = 0..15).
+// v_permlane32_swap(a, b) leaves a = [a.lo32, b.lo32] and b = [a.hi32, b.hi32];
+// v_permlane16_swap(a, b) leaves a = [a.row0, b.row0, a.row2, b.row2] and
+// b = [a.row1, b.row1, a.row3, b.row3] (rows of 16 lanes).
+__device__ __forceinline__ float deliver(float r0, float r1, float r2, float r3)
+{
+    const auto s02 = __builtin_amdgcn_permlane32_swap(__float_as_uint(r0), __float_as_uint(r2), false, false);
+    const auto s13 = __builtin_amdgcn_permlane32_swap(__float_as_uint(r1), __float_as_uint(r3), false, false);
+    // s02[0] = [r0.row0, r0.row1, r2.row0, r2.row1], s13[0] likewise for r1, r3
+    const auto s = __builtin_amdgcn_permlane16_swap(s02[0], s13[0], false, false);
+    return __uint_as_float(s[0]);
+}
+
 // Network outputs for the 64 cells of a wave; lane c passes the features of
 // cell c and receives that cell's outputs (y1 only when BOTH).  All 64 lanes
 // must be active.
 template <bool BOTH>
-__device__ __forceinline__ void net_eval(const NetRegs &nr, const float *x, int lane, float &y0, float &y1)
+__device__ __forceinline__ void net_eval(const NetRegs &nr, const float *x, float &y0, float &y1)
 {
-    const int g = lane >> 4;
     const bf16x4 bx = __builtin_bit_cast(
         bf16x4, ((unsigned long long)cvt_pk_bf16(x[2], x[3]) << 32) | cvt_pk_bf16(x[0], x[1]));
+    f32x4 out[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const f32x4 z1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w1[t], bx, nr.b1, 0, 0, 0);
         const bf16x4 h1 = __builtin_bit_cast(bf16x4, relu_pack4(z1));
         const f32x4 z2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w2, h1, nr.b2, 0, 0, 0);
         const bf16x4 h2 = __builtin_bit_cast(bf16x4, relu_pack4(z2));
-        const f32x4 out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w3, h2, nr.b3, 0, 0, 0);
-        // lanes 0..15 hold the outputs of cells 16 t + lane; deliver to their owners
-        const float v0 = __shfl(out[0], lane & 15);
-        if (t == g) y0 = v0;
-        if constexpr (BOTH) {
-            const float v1 = __shfl(out[1], lane & 15);
-            if (t == g) y1 = v1;
-        }
+        out[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nr.w3, h2, nr.b3, 0, 0, 0);
     }
+    // Row 0 (lanes 0..15) of out[t][0] holds y0 of cells 16 t + lane.  Two
+    // 32-lane swaps put tiles 0 | 2 and 1 | 3 side by side, a 16-lane swap
+    // interleaves them: lane 16 t + n then holds out[t][0] of lane n.
+    y0 = deliver(out[0][0], out[1][0], out[2][0], out[3][0]);
+    if constexpr (BOTH) y1 = deliver(out[0][1], out[1][1], out[2][1], out[3][1]);
 }
 
 // ---------------------------------------------------------------- solve
@@ -468,7 +481,7 @@ __global__ void __launch_bounds__(256, 3) k_solve(SolveArgs a)
         } else {
             const Feat f = c4_features(u[0], u[1], S2);
             float y0 = 0.0f, y1 = 0.0f;
-            net_eval<false>(nr, f.x, lane, y0, y1);
+            net_eval<false>(nr, f.x, y0, y1);
             if (f.ok && __builtin_isfinite(y0)) {
                 W = learned_w0(f, y0);
             } else {
@@ -671,7 +684,7 @@ __global__ void __launch_bounds__(256) k_infer(long n, const float *__restrict__
 #pragma unroll
         for (int k = 0; k < NIN; k++) x[k] = i < n ? X[NIN * i + k] : 0.0f;
         float y0 = 0.0f, y1 = 0.0f;
-        net_eval<true>(nr, x, lane, y0, y1);
+        net_eval<true>(nr, x, y0, y1);
         if (i < n) {
             Y[2 * i] = y0;
             Y[2 * i + 1] = y1;
