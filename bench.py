@@ -519,12 +519,14 @@ def run_c2p(n_cells=1 << 24, n_train=1 << 22, train_steps=20000, reps=5, log=Non
     bytes_cell = 80
     t_s = res["learned"]["ms_median"] * 1e-3
     ach = bytes_cell * n_cells / t_s / 1e9
-    # The solve is VALU-issue bound, not HBM bound: rocprofv3 counts 860 VALU
-    # wave-instructions per 64-cell chunk for the learned guess (2.76 Newton
-    # iterations per cell with the quick network; the full one needs fewer),
-    # profiles/c2p_solve_2e24_counters.md.  A wave64 instruction holds a
-    # 16-lane SIMD for 4 cycles: 256 CUs x 4 SIMDs x 2.4 GHz / 4 per second.
-    issue = 860.0 * (n_cells / 64) / t_s / (256 * 4 * 2.4e9 / 4)
+    # The solve is VALU-issue bound, not HBM bound: rocprofv3 counts 700 VALU
+    # wave-instructions per 64-cell chunk for the learned guess at 4.0 wave-max
+    # Newton iterations (quick network) and 103 per iteration, so ~600 at the
+    # full network's 3.0 (profiles/c2p_solve_2e24_counters.md).  A wave64
+    # instruction holds a 16-lane SIMD for 4 cycles: 256 CUs x 4 SIMDs x
+    # 2.4 GHz / 4 per second.
+    valu_chunk = 700.0 - 103.0 * (4.0 - res["learned"]["wave_max_mean"])
+    issue = valu_chunk * (n_cells / 64) / t_s / (256 * 4 * 2.4e9 / 4)
     out["roofline"] = {"bound": "valu_issue", "kernel": "k_solve<learned>", "bytes_per_cell": bytes_cell,
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                        "valu_issue_frac_est": issue,
