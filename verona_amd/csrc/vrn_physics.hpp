@@ -142,48 +142,91 @@ template <> struct WenoEps<float> { static constexpr float v = 1.0e-30f; };
 // three cells in the same order up to the commutative first sum, so one
 // value serves all three positions bit for bit (the marching sweeps compute
 // it once per cell and carry it).
+#if defined(VRN_FAST)
+// Fast FP64 form: every smoothness indicator is carried scaled by 12/13 with
+// the equally scaled epsilon added, e = (12/13)(beta + 1e-42) =
+// c^2 + (3/13) l^2 + WZ_EPS12 (c the curvature second difference, l the
+// first-difference combination of the stencil), so the curvature term is
+// c^2 + WZ_EPS12 and the ratios tau5 / (beta + eps) are unchanged.
+constexpr double WZ_EPS12 = 12.0 / 13.0 * 1.0e-42;
+#endif
 template <class T> __device__ __forceinline__ T wz_curv(T a, T b, T c)
 {
+#if defined(VRN_FAST)
+    if constexpr (std::is_same<T, double>::value) {
+        const double d = a + c - 2.0 * b;
+        return d * d + WZ_EPS12;
+    }
+#endif
     return T(13.0) / T(12.0) * sq(a + c - T(2) * b);
 }
+
+#if defined(VRN_FAST)
+// Fast FP64 WENO-Z from the scaled indicators above (fluxlimiter.jl:19-71
+// rearranged; equal to it up to rounding):
+//  * e_k = (3/13) l_k^2 + k_k, tau = |e_0 - e_2|; one reciprocal of
+//    e_0 e_1 e_2 (>= WZ_EPS12^3 ~ 8e-127, normal) gives the three ratios
+//    tau / e_k, and s_k = 1 + (tau / e_k)^2;
+//  * the six stencil polynomials share their terms: with A = q_{i-1} + 5 q_i
+//    and B = q_{i+1} + 5 q_i, f0 = 2 l_0 + A, f2 = B - l_2 (face i+1/2) and
+//    f0' = 2 l_2 + B, f2' = A - l_0, f1' = f1 + 3 l_1 (face i-1/2);
+//  * the linear weights (1, 6, 3)/10 enter both numerator and denominator,
+//    and both faces are normalised with one reciprocal of the product of
+//    their denominators (each >= 10, finite unless a stencil jump exceeds
+//    ~1e17).
+__device__ __forceinline__ void wenoz_fast(double qm2, double qm1, double q, double qp1, double qp2, double k0,
+                                           double k1, double k2, double &lo, double &hi)
+{
+    constexpr double c313 = 3.0 / 13.0;
+    const double l0 = (qm2 + 3.0 * q) - 4.0 * qm1;
+    const double l1 = qm1 - qp1;
+    const double l2 = (qp2 + 3.0 * q) - 4.0 * qp1;
+    const double e0 = c313 * (l0 * l0) + k0;
+    const double e1 = c313 * (l1 * l1) + k1;
+    const double e2 = c313 * (l2 * l2) + k2;
+    const double tau = dabs(e0 - e2);
+    const double e01 = e0 * e1;
+    const double r = tau * frcp(e01 * e2);
+    const double i0 = r * (e1 * e2), i1 = r * (e0 * e2), i2 = r * e01;
+    const double s0 = i0 * i0 + 1.0, s1 = i1 * i1 + 1.0, s2 = i2 * i2 + 1.0;
+    const double q5 = 5.0 * q;
+    const double A = q5 + qm1, B = q5 + qp1;
+    const double f0h = 2.0 * l0 + A;
+    const double f1h = 2.0 * qp1 + (q5 - qm1);
+    const double f2h = B - l2;
+    const double f0l = 2.0 * l2 + B;
+    const double f1l = 3.0 * l1 + f1h;
+    const double f2l = A - l0;
+    const double z1 = 6.0 * s1, z2 = 3.0 * s2, z0 = 3.0 * s0;
+    const double nh = s0 * f0h + z1 * f1h + z2 * f2h, dh = s0 + z1 + z2;
+    const double nl = s2 * f0l + z1 * f1l + z0 * f2l, dl = s2 + z1 + z0;
+    const double r6 = frcp(dh * dl) * (1.0 / 6.0);
+    hi = nh * (dl * r6);
+    lo = nl * (dh * r6);
+}
+#endif
 
 // fluxlimiter.jl:19-71  WENOZ(q_{i-2..i+2}) -> lo = q at i-1/2, hi = q at i+1/2,
 // from the five values and the curvature terms k0 = wz_curv(q_{i-2}, q_{i-1},
 // q_i), k1 = wz_curv(q_{i-1}, q_i, q_{i+1}), k2 = wz_curv(q_i, q_{i+1}, q_{i+2}).
-// SHARED_RCP (fast FP64 form): both faces' normalisations from one
-// reciprocal -- fewer instructions, four more live registers per call.
-template <bool SHARED_RCP = false, class T>
+template <class T>
 __device__ __forceinline__ void wenoz_k(T qm2, T qm1, T q, T qp1, T qp2, T k0, T k1, T k2, T &lo, T &hi)
 {
+#if defined(VRN_FAST)
+    if constexpr (std::is_same<T, double>::value) {
+        wenoz_fast(qm2, qm1, q, qp1, qp2, k0, k1, k2, lo, hi);
+        return;
+    }
+#endif
     const T b2c = T(0.25);
     T beta0 = k0 + b2c * sq(qm2 + T(3) * q - T(4) * qm1);
     T beta1 = k1 + b2c * sq(qm1 - qp1);
     T beta2 = k2 + b2c * sq(qp2 + T(3) * q - T(4) * qp1);
     const T eps = WenoEps<T>::v;
     T tau5 = dabs(beta0 - beta2);
-    // beta + eps >= 1e-42, a normal double, so the fast reciprocal is safe here
-#if defined(VRN_FAST)
-    T ind0, ind1, ind2;
-    if constexpr (std::is_same<T, double>::value) {
-        // one division for the three indicators: the product of the three
-        // (beta + eps) is >= 1e-126 (normal) and far from overflow for
-        // physical data
-        const T e0 = beta0 + eps, e1 = beta1 + eps, e2 = beta2 + eps;
-        const T e01 = e0 * e1;
-        const T r = fdiv(tau5, e01 * e2);
-        ind0 = r * (e1 * e2);
-        ind1 = r * (e0 * e2);
-        ind2 = r * e01;
-    } else {
-        ind0 = fdiv(tau5, beta0 + eps);
-        ind1 = fdiv(tau5, beta1 + eps);
-        ind2 = fdiv(tau5, beta2 + eps);
-    }
-#else
     T ind0 = fdiv(tau5, beta0 + eps);
     T ind1 = fdiv(tau5, beta1 + eps);
     T ind2 = fdiv(tau5, beta2 + eps);
-#endif
     T s0, s1, s2;
     if constexpr (std::is_same<T, float>::value) {
         // FP32 form: ind^2 overflows FP32 once ind > ~1.8e19 (a flat side next
@@ -208,12 +251,7 @@ __device__ __forceinline__ void wenoz_k(T qm2, T qm1, T q, T qp1, T qp2, T k0, T
     T f1 = -qm1 + T(5) * q + T(2) * qp1;
     T f2 = T(2) * q + T(5) * qp1 - qp2;
     const T nh = f0 * a0 + f1 * a1 + f2 * a2, dh = T(6) * (a0 + a1 + a2);
-#if defined(VRN_FAST)
-    constexpr bool share = SHARED_RCP && std::is_same<T, double>::value;
-#else
-    constexpr bool share = false;
-#endif
-    if constexpr (!share) hi = fdiv(nh, dh);
+    hi = fdiv(nh, dh);
 
     a0 = T(0.1) * s2;
     a2 = T(0.3) * s0;
@@ -221,23 +259,13 @@ __device__ __forceinline__ void wenoz_k(T qm2, T qm1, T q, T qp1, T qp2, T k0, T
     f1 = -qp1 + T(5) * q + T(2) * qm1;
     f2 = T(2) * q + T(5) * qm1 - qm2;
     const T nl = f0 * a0 + f1 * a1 + f2 * a2, dl = T(6) * (a0 + a1 + a2);
-    if constexpr (share) {
-        // one reciprocal for both faces: each denominator is >= 6 (every
-        // s >= 1) and their product stays finite unless a stencil jump
-        // exceeds ~1e17
-        const T r = frcp(dh * dl);
-        hi = nh * (dl * r);
-        lo = nl * (dh * r);
-    } else {
-        lo = fdiv(nl, dl);
-    }
+    lo = fdiv(nl, dl);
 }
 
-template <bool SHARED_RCP = false, class T>
+template <class T>
 __device__ __forceinline__ void wenoz(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
 {
-    wenoz_k<SHARED_RCP>(qm2, qm1, q, qp1, qp2, wz_curv(qm2, qm1, q), wz_curv(qm1, q, qp1), wz_curv(q, qp1, qp2),
-                        lo, hi);
+    wenoz_k(qm2, qm1, q, qp1, qp2, wz_curv(qm2, qm1, q), wz_curv(qm1, q, qp1), wz_curv(q, qp1, qp2), lo, hi);
 }
 
 // fluxlimiter.jl:1-13  MINMOD(q_{i-1}, q_i, q_{i+1})
@@ -341,11 +369,11 @@ __device__ __forceinline__ void ppm(T q_im2, T q_im1, T q_i, T q_ip1, T q_ip2, T
     hi = ql_ip1;
 }
 
-template <int REC, bool SHARED_RCP = false, class T>
+template <int REC, class T>
 __device__ __forceinline__ void recon(T qm2, T qm1, T q, T qp1, T qp2, T &lo, T &hi)
 {
     if constexpr (REC == REC_WENOZ)
-        wenoz<SHARED_RCP>(qm2, qm1, q, qp1, qp2, lo, hi);
+        wenoz(qm2, qm1, q, qp1, qp2, lo, hi);
     else if constexpr (REC == REC_PPM)
         ppm(qm2, qm1, q, qp1, qp2, lo, hi);
     else

@@ -143,10 +143,10 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const d
 // face c-1/2 (c >= x0) and updates cell c-1 (c >= x0+1).  Whole rows are
 // x0 = 3, x1 = NX-3.
 // Waves per SIMD the x sweep is compiled for: the fast FP64 WENO-Z
-// instantiations fit 128 VGPRs without spilling and run ~3 % faster at 4 than
-// at the unbounded 3; the others would spill, so they stay unbounded.
+// instantiations take 131 VGPRs at 3 (384^3 jet: 1.68 ms per launch; bounded
+// to 4 they spill 12 bytes and take 1.75 ms); the others stay unbounded.
 #if defined(VRN_FAST)
-template <class R, int REC> constexpr int xsweep_waves() { return REC == REC_WENOZ ? 4 : 1; }
+template <class R, int REC> constexpr int xsweep_waves() { return REC == REC_WENOZ ? 3 : 1; }
 #else
 template <class R, int REC> constexpr int xsweep_waves() { return 1; }
 #endif
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             const double *q = P + v * ps + rbu + c;
-            recon<REC, true>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
+            recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
         }
         lo[0] = floor_max(fl, lo[0]);
         lo[1] = floor_max(fl, lo[1]);
@@ -352,17 +352,15 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         R lo[NV], up[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            // the last sweep (RK combine + con2prim) has no registers to spare
             if constexpr (CARRY) {
                 const R q2 = wl[(s2 * NV + v) * WS], q3 = wl[(s3 * NV + v) * WS], q4 = wl[(s4 * NV + v) * WS];
                 R *kp = kl + (((c + 1) & 1) * NV + v) * WS;  // cell c-1's term, then cell c+1's
                 const R km = *kp, k0 = kl[((c & 1) * NV + v) * WS], kn = wz_curv(q2, q3, q4);
                 *kp = kn;
-                wenoz_k<!LAST>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], q2, q3, q4, km, k0, kn, lo[v],
-                               up[v]);
+                wenoz_k(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], q2, q3, q4, km, k0, kn, lo[v], up[v]);
             } else {
-                recon<REC, !LAST>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
-                                  wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
+                recon<REC>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
+                           wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
             }
         }
         lo[0] = floor_max(fl, lo[0]);

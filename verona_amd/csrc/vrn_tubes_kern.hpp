@@ -147,7 +147,7 @@ __device__ void tube_stage(const vrn::TubeArgs &a, int tb, int s, double *lds)
                 if (REC == REC_MINMOD)
                     minmod(q[-1], q[0], q[1], lo, hi);
                 else
-                    recon<REC, true>(q[-2], q[-1], q[0], q[1], q[2], lo, hi);
+                    recon<REC>(q[-2], q[-1], q[0], q[1], q[2], lo, hi);
                 PLc[v] = hi;
                 PRm[v] = lo;
             }
