@@ -113,11 +113,13 @@ def test_fast_fp64_form2(oracle):
     assert max(errs) <= TOL
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64fast", "fp32"])
 def test_kelvin_helmholtz(oracle, precision):
     """The 2D Kelvin-Helmholtz problem (periodic, WENOZ + HLLC + SSP-RK3, CFL
-    0.4) at 128^2: bit-level parity in FP64 after 20 steps; the FP32
-    reconstruction/Riemann form's error against it is reported and bounded."""
+    0.4) at 128^2: parity with the FP64 oracle after 20 steps, to the FP64
+    tolerance in both FP64 forms (the fast one solves the 4x4 Newton system
+    in closed form); the FP32 reconstruction/Riemann form's error against it
+    is reported and bounded."""
     from verona_amd import DeviceGrid
     from verona_amd.problems import kelvin_helmholtz
     progress(f"test_kelvin_helmholtz {precision}")
@@ -142,7 +144,7 @@ def test_kelvin_helmholtz(oracle, precision):
     assert bad == 0 and t > 0
     errs = per_var_rel_l1(interior(Pg), interior(from_oracle2(Po)))
     progress(f"  per-var rel L1 vs FP64 oracle {['%.2e' % e for e in errs]}")
-    if precision == "fp64":
+    if precision != "fp32":
         assert max(errs) <= TOL
     else:
         # u^y is the 1 % transverse perturbation, so its relative error is

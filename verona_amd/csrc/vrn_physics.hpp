@@ -897,129 +897,195 @@ __device__ __noinline__ int c2p_cascade_tail(const double *U, double *P, double 
     return conv2 ? 2 : (conv5 ? 3 : 4);
 }
 
+// One Newton step of the reference's 4x4 con2prim (2dim/structs.jl:249-330):
+// residual f and Jacobian A of P -> U at P, LU with row-scaled partial
+// pivoting and explicit fma (:121-208), o = A^-1 f.  Returns false if a row
+// of A is identically zero (the reference raises).
+__device__ __forceinline__ bool newton_step_2d_lu(const double *U, const double *P, double g, double *o)
+{
+    double gam = vrn_sqrt1(P[2] * P[2] + P[3] * P[3] + 1.0);
+    double w = g * P[1] + P[0];
+    double f[4], A[16], rn[4];
+#if defined(VRN_FAST)
+    // fast form: one reciprocal of gamma, of each row maximum and of each
+    // pivot instead of the 19 divisions per iteration
+    const double ig = recip(gam);
+    double rp[4];
+#endif
+    f[0] = gam * P[0] - U[0];
+    f[1] = (g - 1.0) * P[1] - gam * gam * w - U[1];
+    f[2] = P[2] * gam * w - U[2];
+    f[3] = P[3] * gam * w - U[3];
+    A[0] = gam;
+    A[4] = 0.0;
+#if defined(VRN_FAST)
+    A[8] = P[0] * P[2] * ig;
+    A[12] = P[0] * P[3] * ig;
+#else
+    A[8] = P[0] * P[2] / gam;
+    A[12] = P[0] * P[3] / gam;
+#endif
+    A[1] = -(gam * gam);
+    A[5] = (g - 1.0) - gam * gam * g;
+    A[9] = -2.0 * P[2] * w;
+    A[13] = -2.0 * P[3] * w;
+    A[2] = gam * P[2];
+    A[6] = gam * P[2] * g;
+#if defined(VRN_FAST)
+    A[10] = (2.0 * (P[2] * P[2]) + P[3] * P[3] + 1.0) * ig * w;
+    A[14] = P[2] * P[3] * ig * w;
+#else
+    A[10] = (2.0 * (P[2] * P[2]) + P[3] * P[3] + 1.0) / gam * w;
+    A[14] = P[2] * P[3] / gam * w;
+#endif
+    A[3] = gam * P[3];
+    A[7] = gam * P[3] * g;
+#if defined(VRN_FAST)
+    A[11] = P[2] * P[3] * ig * w;
+    A[15] = (2.0 * (P[3] * P[3]) + P[2] * P[2] + 1.0) * ig * w;
+#else
+    A[11] = P[2] * P[3] / gam * w;
+    A[15] = (2.0 * (P[3] * P[3]) + P[2] * P[2] + 1.0) / gam * w;
+#endif
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double am = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double t = dabs(A[j * 4 + i]);
+            // `if (t > am) am = t` of the reference: am starts at +0 and
+            // t = |A| is >= +0 or a quiet NaN, which both forms skip, so
+            // IEEE maxNum gives the same value in one instruction
+            am = __builtin_fmax(am, t);
+        }
+        if (am == 0.0) return false;
+        rn[i] = recip(am);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int piv = k;
+        double pv = dabs(A[k * 4 + k]) * rn[k];
+#pragma unroll
+        for (int i = k + 1; i < 4; i++) {
+            double t = dabs(A[k * 4 + i]) * rn[i];
+            if (t > pv) {
+                pv = t;
+                piv = i;
+            }
+        }
+        // Row swap written as selects so the arrays stay in registers.
+        // The right-hand side is eliminated along with the matrix (the
+        // forward substitution's fma(-L_ij, x_j, x_i) for j = 0..i-1, in
+        // that order, as each step finishes), so the multipliers of the
+        // columns before k are dead and only columns k..3 move; rn[k] is
+        // not read after this step.
+#pragma unroll
+        for (int i = k + 1; i < 4; i++) {
+            bool s = (piv == i);
+#pragma unroll
+            for (int j = k; j < 4; j++) {
+                double a = A[j * 4 + k], b = A[j * 4 + i];
+                A[j * 4 + k] = s ? b : a;
+                A[j * 4 + i] = s ? a : b;
+            }
+            double a = f[k], b = f[i];
+            f[k] = s ? b : a;
+            f[i] = s ? a : b;
+            rn[i] = s ? rn[k] : rn[i];
+        }
+#if defined(VRN_FAST)
+        rp[k] = recip(A[k * 4 + k]);
+#endif
+#pragma unroll
+        for (int i = k + 1; i < 4; i++) {
+#if defined(VRN_FAST)
+            const double m = A[k * 4 + i] * rp[k];
+#else
+            const double m = A[k * 4 + i] / A[k * 4 + k];
+#endif
+#pragma unroll
+            for (int j = k + 1; j < 4; j++) A[j * 4 + i] = __builtin_fma(-m, A[j * 4 + k], A[j * 4 + i]);
+            f[i] = __builtin_fma(-m, f[k], f[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 3; i >= 0; i--) {
+        o[i] = f[i];
+#pragma unroll
+        for (int j = i + 1; j < 4; j++) o[i] = __builtin_fma(-A[j * 4 + i], o[j], o[i]);
+#if defined(VRN_FAST)
+        o[i] *= rp[i];
+#else
+        o[i] /= A[i * 4 + i];
+#endif
+    }
+    return true;
+}
+
+#if defined(VRN_FAST)
+// Fast FP64 form: the same Newton step solved in closed form.  With
+// c = (g-1)/g, q2 = ux^2 + uy^2 = Gamma^2 - 1, a = d_rho + g d_u and
+// s = ux d_ux + uy d_uy, rows 2 and 3 of A o = f read
+//   Gamma u_i a + (w/Gamma)(Gamma^2 d_ui + u_i s) = f_i,
+// row 0 gives d_rho = (f0 - rho s/Gamma)/Gamma, and ux * row 2 + uy * row 3
+// together with row 1 leave the 2x2 system
+//   (c - Gamma^2) a + (c rho/Gamma^2 - 2w) s = f1 + c f0/Gamma
+//   Gamma q2 a + (w/Gamma)(2 Gamma^2 - 1) s = ux f2 + uy f3
+// whose determinant is Dn/Gamma with
+//   Dn = w ((2c - 1) Gamma^2 - c) - c rho q2 <= -c w < 0
+// for 1 < g <= 2, rho > 0, w > 0 (the Gamma^4 terms cancel algebraically, so
+// there is no cancellation in floating point).  Then d_u = (a - d_rho)/g and
+// d_ui = (Gamma (f_i - Gamma u_i a)/w - u_i s)/Gamma^2.  On such states no row
+// of A is zero (row 0 holds Gamma >= 1, row 1 -Gamma^2, row 2 Gamma ux or
+// Gamma w), so the reference's singular exit cannot occur.  Against a 50-digit
+// solve of the same systems it is as accurate as the pivoted LU (median
+// 7e-14 vs 2e-13, p99 1e-8 vs 3e-8 component-wise relative, Gamma up to 200).
+__device__ __forceinline__ void newton_step_2d_closed(const double *U, const double *P, double g, double c,
+                                                      double ig_g, double q2, double g2, double w, double *o)
+{
+    const double rho = P[0], x = P[2], y = P[3];
+    const double gam = vrn_sqrt1(g2);
+    const double ig = recip(gam);
+    const double f0 = gam * rho - U[0];
+    const double f1 = (g - 1.0) * P[1] - g2 * w - U[1];
+    const double f2 = x * gam * w - U[2];
+    const double f3 = y * gam * w - U[3];
+    const double r1 = f1 + c * f0 * ig;
+    const double r2 = x * f2 + y * f3;
+    const double iD = recip(w * ((2.0 * c - 1.0) * g2 - c) - c * rho * q2);
+    const double a = (r1 * w * (2.0 * g2 - 1.0) + (2.0 * w * gam - c * rho * ig) * r2) * iD;
+    const double s = gam * ((c - g2) * r2 - gam * q2 * r1) * iD;
+    const double iw = recip(w), ig2 = ig * ig;
+    o[0] = (f0 - rho * ig * s) * ig;
+    o[1] = (a - o[0]) * ig_g;
+    o[2] = (gam * (f2 - gam * x * a) * iw - x * s) * ig2;
+    o[3] = (gam * (f3 - gam * y * a) * iw - y * s) * ig2;
+}
+#endif
+
 // 2dim/structs.jl:249-330  4x4 Newton on (rho, u, ux, uy), warm start in P,
-// LU with row-scaled partial pivoting and explicit fma (:121-208).  Returns
-// false if a row of the Jacobian is identically zero (the reference raises).
+// break before the update when |o| < tol.  Returns false if a row of the
+// Jacobian is identically zero (the reference raises).  The fast FP64 form
+// takes the closed-form step on regular states (1 < g <= 2, rho > 0,
+// w = g u + rho > 1e-300, Gamma^2 w < 1e300) and the LU elsewhere.
 __device__ __forceinline__ bool c2p_2d(const double *U, double *P, double g, int n_iter, double tol,
                                        int &iters)
 {
+#if defined(VRN_FAST)
+    const double c = (g - 1.0) / g, ig_g = 1.0 / g;
+#endif
     for (int it = 0; it < n_iter; it++) {
         iters++;
-        double gam = vrn_sqrt1(P[2] * P[2] + P[3] * P[3] + 1.0);
-        double w = g * P[1] + P[0];
-        double f[4], A[16], o[4], rn[4];
+        double o[4];
 #if defined(VRN_FAST)
-        // fast form: one reciprocal of gamma, of each row maximum and of each
-        // pivot instead of the 19 divisions per iteration
-        const double ig = recip(gam);
-        double rp[4];
+        const double q2 = P[2] * P[2] + P[3] * P[3], g2 = q2 + 1.0;
+        const double w = g * P[1] + P[0];
+        if (g <= 2.0 && P[0] > 0.0 && w > 1e-300 && g2 * w < 1e300) {
+            newton_step_2d_closed(U, P, g, c, ig_g, q2, g2, w, o);
+        } else
 #endif
-        f[0] = gam * P[0] - U[0];
-        f[1] = (g - 1.0) * P[1] - gam * gam * w - U[1];
-        f[2] = P[2] * gam * w - U[2];
-        f[3] = P[3] * gam * w - U[3];
-        A[0] = gam;
-        A[4] = 0.0;
-#if defined(VRN_FAST)
-        A[8] = P[0] * P[2] * ig;
-        A[12] = P[0] * P[3] * ig;
-#else
-        A[8] = P[0] * P[2] / gam;
-        A[12] = P[0] * P[3] / gam;
-#endif
-        A[1] = -(gam * gam);
-        A[5] = (g - 1.0) - gam * gam * g;
-        A[9] = -2.0 * P[2] * w;
-        A[13] = -2.0 * P[3] * w;
-        A[2] = gam * P[2];
-        A[6] = gam * P[2] * g;
-#if defined(VRN_FAST)
-        A[10] = (2.0 * (P[2] * P[2]) + P[3] * P[3] + 1.0) * ig * w;
-        A[14] = P[2] * P[3] * ig * w;
-#else
-        A[10] = (2.0 * (P[2] * P[2]) + P[3] * P[3] + 1.0) / gam * w;
-        A[14] = P[2] * P[3] / gam * w;
-#endif
-        A[3] = gam * P[3];
-        A[7] = gam * P[3] * g;
-#if defined(VRN_FAST)
-        A[11] = P[2] * P[3] * ig * w;
-        A[15] = (2.0 * (P[3] * P[3]) + P[2] * P[2] + 1.0) * ig * w;
-#else
-        A[11] = P[2] * P[3] / gam * w;
-        A[15] = (2.0 * (P[3] * P[3]) + P[2] * P[2] + 1.0) / gam * w;
-#endif
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            double am = 0.0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                double t = dabs(A[j * 4 + i]);
-                // `if (t > am) am = t` of the reference: am starts at +0 and
-                // t = |A| is >= +0 or a quiet NaN, which both forms skip, so
-                // IEEE maxNum gives the same value in one instruction
-                am = __builtin_fmax(am, t);
-            }
-            if (am == 0.0) return false;
-            rn[i] = recip(am);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            int piv = k;
-            double pv = dabs(A[k * 4 + k]) * rn[k];
-#pragma unroll
-            for (int i = k + 1; i < 4; i++) {
-                double t = dabs(A[k * 4 + i]) * rn[i];
-                if (t > pv) {
-                    pv = t;
-                    piv = i;
-                }
-            }
-            // Row swap written as selects so the arrays stay in registers.
-            // The right-hand side is eliminated along with the matrix (the
-            // forward substitution's fma(-L_ij, x_j, x_i) for j = 0..i-1, in
-            // that order, as each step finishes), so the multipliers of the
-            // columns before k are dead and only columns k..3 move; rn[k] is
-            // not read after this step.
-#pragma unroll
-            for (int i = k + 1; i < 4; i++) {
-                bool s = (piv == i);
-#pragma unroll
-                for (int j = k; j < 4; j++) {
-                    double a = A[j * 4 + k], b = A[j * 4 + i];
-                    A[j * 4 + k] = s ? b : a;
-                    A[j * 4 + i] = s ? a : b;
-                }
-                double a = f[k], b = f[i];
-                f[k] = s ? b : a;
-                f[i] = s ? a : b;
-                rn[i] = s ? rn[k] : rn[i];
-            }
-#if defined(VRN_FAST)
-            rp[k] = recip(A[k * 4 + k]);
-#endif
-#pragma unroll
-            for (int i = k + 1; i < 4; i++) {
-#if defined(VRN_FAST)
-                const double m = A[k * 4 + i] * rp[k];
-#else
-                const double m = A[k * 4 + i] / A[k * 4 + k];
-#endif
-#pragma unroll
-                for (int j = k + 1; j < 4; j++) A[j * 4 + i] = __builtin_fma(-m, A[j * 4 + k], A[j * 4 + i]);
-                f[i] = __builtin_fma(-m, f[k], f[i]);
-            }
-        }
-#pragma unroll
-        for (int i = 3; i >= 0; i--) {
-            o[i] = f[i];
-#pragma unroll
-            for (int j = i + 1; j < 4; j++) o[i] = __builtin_fma(-A[j * 4 + i], o[j], o[i]);
-#if defined(VRN_FAST)
-            o[i] *= rp[i];
-#else
-            o[i] /= A[i * 4 + i];
-#endif
+        if (!newton_step_2d_lu(U, P, g, o)) {
+            return false;
         }
 #if defined(VRN_FAST)
         if (o[0] * o[0] + o[1] * o[1] + o[2] * o[2] + o[3] * o[3] < tol * tol) break;

@@ -407,13 +407,15 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 }
 
 // ------------------------------------------------------------ 2D stage end
-// In 2D the y sweep leaves the stage's accumulated update in Uacc, like the
-// middle sweep in 3D, and this pointwise pass forms the RK combination and
-// runs the 2D Newton con2prim (2dim/structs.jl:249-330) from the stage's
-// primitives, Limit and the CFL rate -- the same arithmetic as doing it
-// inside the march.  The 4x4 Newton needs ~200 registers: inside the march
-// that left 2 waves per SIMD with one wave per 64-column segment (2048 waves
-// at 2048^2); here the cells are spread over every SIMD.
+// Exact FP64 and FP32 forms: the y sweep leaves the stage's accumulated
+// update in Uacc, like the middle sweep in 3D, and this pointwise pass forms
+// the RK combination and runs the 2D Newton con2prim (2dim/structs.jl:249-330)
+// from the stage's primitives, Limit and the CFL rate -- the same arithmetic
+// as doing it inside the march.  With the reference's pivoted LU the Newton
+// step is heavy enough that spreading the cells over every SIMD pays.  The
+// fast FP64 form solves the Newton system in closed form and runs the stage
+// end inside the y march instead (launch_stage), which saves the Uacc round
+// trip and a read of P.
 template <int NV>
 __global__ __launch_bounds__(256) void k_finish2d(SweepArgs a)
 {
@@ -512,6 +514,15 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
         SweepArgs ay = a;
         ay.dx_dim = dxs[1];
         int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
+#if defined(VRN_FAST)
+        // fast FP64 form: the y sweep is the last sweep (RK combination,
+        // closed-form 4x4 Newton, Limit and rate inside the march); same
+        // results as the separate pass, bit for bit
+        timer(tctx, 1, 0);
+        hipLaunchKernelGGL((k_march<4, 1, R, REC, RIEM, true>), dim3(nxb, 1, nseg), dim3(64), 0, s, ay);
+        timer(tctx, 1, 1);
+        return;
+#endif
         timer(tctx, 1, 0);
         hipLaunchKernelGGL((k_march<4, 1, R, REC, RIEM, false>), dim3(nxb, 1, nseg), dim3(64), 0, s, ay);
         timer(tctx, 1, 1);
