@@ -373,9 +373,10 @@ def run_c1_gpu():
     with Tubes([n]) as tb:
         tb.upload(rho, u, ux, [0.4 * dx], [dx])
         tb.set_method("minmod", "hllc", 50, 1e-10)
-        tb.run(steps, 128, 1)  # graph capture + warm-up
+        # one workgroup of 1024 lanes: each stage is one 1024-cell chunk
+        tb.run(steps, 1024, 1)  # graph capture + warm-up
         tb.upload(rho, u, ux, [0.4 * dx], [dx])
-        ms = tb.run(steps, 128, 1)
+        ms = tb.run(steps, 1024, 1)
     return {"what": "1D shock tube N=1024, minmod+HLLC, RK2, to t=0.4, one tube (one workgroup)",
             "steps": steps, "ms": ms, "value": n * steps / (ms * 1e-3), "unit": "cell-steps/s"}
 

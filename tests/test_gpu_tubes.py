@@ -109,7 +109,7 @@ def test_tubes_refuse_fp32():
 def test_graph_replay_and_autotune_are_transparent(oracle):
     """Graph replays (several launches in a row) and an autotune pass (which
     checkpoints and restores the state) give the same answer as one eager run."""
-    from verona_amd.tubes import Tubes, shock_tube_batch
+    from verona_amd.tubes import BLOCKS, Tubes, shock_tube_batch
     progress("test_graph_replay_and_autotune_are_transparent")
     lengths, rho, u, ux, dt, dx = shock_tube_batch(96, lmin=32, lmax=400, seed=9)
     res = []
@@ -124,7 +124,7 @@ def test_graph_replay_and_autotune_are_transparent(oracle):
                     tb.run(8, 128, 12, use_graph=True)
             else:
                 shape = tb.autotune(probe_steps=1)
-                assert shape[0] in (64, 128, 256) and shape[1] >= 1
+                assert shape[0] in BLOCKS and shape[1] >= 1
                 tb.run(24, *shape)
             res.append(np.stack(tb.download()))
     assert np.array_equal(res[0], res[1], equal_nan=True)

@@ -39,7 +39,7 @@ TUBE_SIGNATURES = {
 }
 _lib.SIGNATURES.update(TUBE_SIGNATURES)
 
-BLOCKS = (64, 128, 256)
+BLOCKS = (64, 128, 256, 512, 1024)
 WAVES_PER_CU = (1, 2, 4, 8, 12, 16, 20)
 _CACHE = {}
 
