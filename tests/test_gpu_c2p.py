@@ -6,7 +6,8 @@
   (pins the lane layout), close to the NumPy BF16 model otherwise;
 * training-kernel gradients against the NumPy reference (FP32 and QAT);
 * training: deterministic, the loss falls, and the trained guess needs fewer
-  Newton iterations than the analytic and generic starts while the
+  Newton iterations than the analytic and generic starts (and less solve
+  time than the generic start) while the
   recovered P agrees with them to 1e-12 (relative L1 per variable) and with
   the oracle bit for bit from the same initial W; no unrecovered cells.
 """
