@@ -568,19 +568,26 @@ def measured_traffic(kernel, n, precision):
     return None if k is None else float(k["bytes"])
 
 
+# The kernels of one RK stage of the fused step, by arithmetic form: the fast
+# FP64 form runs the x direction inside the y march (k_xfaces solves the x
+# faces at the marches' wave edges first), the other forms a separate x sweep.
+STAGE_KERNELS = {"fp64fast": ("xfaces", "xysweep", "zsweep_c2p"), "fp64": ("xsweep", "ysweep", "zsweep_c2p")}
+
+
 def measured_step_traffic(n, precision):
-    """Measured HBM bytes per zone-update of the whole step: the three sweeps'
+    """Measured HBM bytes per zone-update of the whole step: the stage kernels'
     calibrated bytes per interior cell per launch, each launched once per RK
     stage, times 3 stages (the rescue and ghost passes are < 1 % of it)."""
     kern = _traffic_profile(n, precision)
-    if not kern or not all(k in kern for k in ("xsweep", "ysweep", "zsweep_c2p")):
+    ks = STAGE_KERNELS.get(precision, ())
+    if not kern or not all(k in kern for k in ks):
         return None
-    return 3.0 * sum(kern[k]["per_cell"] for k in ("xsweep", "ysweep", "zsweep_c2p"))
+    return 3.0 * sum(kern[k]["per_cell"] for k in ks)
 
 
 def valu_issue_ceiling(n, precision):
     """Zone-updates/s the step's instruction count allows at full VALU issue:
-    the three sweeps' VALU lane-instructions per interior cell per launch
+    the stage kernels' VALU lane-instructions per interior cell per launch
     (committed rocprofv3 SQ_INSTS_VALU pass), three launches each per step,
     against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction."""
     path = os.path.join(ROOT, "profiles", "jet384_sweeps_instruction_mix.json")
@@ -588,9 +595,10 @@ def valu_issue_ceiling(n, precision):
         return None
     with open(path) as f:
         mix = json.load(f)["per_cell_lane_instructions"]
-    if not all(k in mix for k in ("xsweep", "ysweep", "zsweep_c2p")):
+    ks = STAGE_KERNELS[precision]
+    if not all(k in mix for k in ks):
         return None
-    lane_ops = 3.0 * sum(mix[k]["SQ_INSTS_VALU"] for k in ("xsweep", "ysweep", "zsweep_c2p"))
+    lane_ops = 3.0 * sum(mix[k]["SQ_INSTS_VALU"] for k in ks)
     return {"valu_lane_ops_per_zone_update": lane_ops,
             "zone_updates_per_s": 256 * 4 * 2.4e9 / 4 * 64 / lane_ops}
 
@@ -675,8 +683,10 @@ def main():
     # The z sweep closes each RK stage, so it carries the stage's compulsory
     # traffic: 560 B per zone-update over 3 stages (stage 1: read P, U^n, write
     # U, P = 160 B; stages 2-3 also read U^n = 200 B; mean 560/3 B per launch).
-    # The x and y sweeps' own contract is read P, read U_b/acc, write acc = 120 B.
-    per_cell = {"zsweep_c2p": BYTES_PER_ZU["rk3"] / 3.0, "xsweep": 3 * 40, "ysweep": 3 * 40}.get(dom, 4 * 40)
+    # The x and y sweeps' own contract is read P, read U_b/acc, write acc =
+    # 120 B; so is the fused x-y march's (read P and U_b, write acc).
+    per_cell = {"zsweep_c2p": BYTES_PER_ZU["rk3"] / 3.0, "xsweep": 3 * 40, "ysweep": 3 * 40,
+                "xysweep": 3 * 40}.get(dom, 4 * 40)
     ach = per_cell * cells / (mean_ms * 1e-3) / 1e9
     step_gbs = BYTES_PER_ZU["rk3"] * value / 1e9
     step_traffic = measured_step_traffic(n, args.precision)

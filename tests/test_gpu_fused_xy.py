@@ -1,0 +1,62 @@
+"""The 3D fast FP64 form's x direction inside the y march (k_xfaces + the XY
+march) against the same form with a separate x sweep: every face the x
+sweep solves is solved by the march (interior faces in-wave, the faces at
+its 64-column wave edges by k_xfaces) with the same operations, and the
+updates are applied in the same order, so the two give the same bits.
+Grids with ragged x extents cover partial last waves, a last wave of one
+active lane and extents that are multiples of 64; the oracle comparison of
+the fused form itself is in test_gpu_parity3d.py (the fast form's default)."""
+import numpy as np
+import pytest
+
+from helpers import interior, progress, state3
+
+pytestmark = pytest.mark.gpu
+G = 4.0 / 3.0
+
+
+def _run(P, fused, steps, *, rec="wenoz", riemann="hllc", integrator="rk3", ref=False, bc=None):
+    from verona_amd import DeviceGrid
+    nx, ny, nz = (s - 6 for s in P.shape[1:])
+    dx, dy, dz = 1 / nx, 1 / ny, 1 / nz
+    bc = bc or ["outflow", "outflow", "periodic", "periodic", "outflow", "outflow"]
+    with DeviceGrid(3, nx, ny, nz, G) as dev:
+        dev.set_xy_fusion(fused)
+        dev.upload("P", P)
+        dev.fill_ghosts(bc)
+        dev.prim_to_cons()
+        if ref:
+            dt = 0.4 / (1 / dx + 1 / dy + 1 / dz)
+            dev.set_step(ref_mode=True, rec=rec, riemann=riemann, integrator=integrator, precision="fp64fast",
+                         n_iter=50, tol=1e-10, floor=1e-7, dt=dt, dx=dx, dy=dy, dz=dz, bc=bc)
+        else:
+            dev.set_step(ref_mode=False, rec=rec, riemann=riemann, integrator=integrator, precision="fp64fast",
+                         n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dy, dz=dz, bc=bc)
+        ts = [dev.step(want_dt=not ref) for _ in range(steps)]
+        out = dev.download("P")
+        nb = dev.counters()[0]
+    return out, ts, nb
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(40, 24, 20), (65, 20, 18), (128, 16, 12), (70, 33, 17), (64, 9, 11)])
+def test_fused_xy_equals_separate_x_sweep(nx, ny, nz):
+    progress(f"test_fused_xy_equals_separate_x_sweep {nx}x{ny}x{nz}")
+    P = state3(nx, ny, nz, seed=21)
+    A, ta, na = _run(P, True, 3)
+    B, tb, nbb = _run(P, False, 3)
+    diff = np.abs(interior(A) - interior(B)).max()
+    progress(f"  max |fused - separate| {diff:.3e}, dt {ta[-1]:.6e} / {tb[-1]:.6e}")
+    assert na == nbb == 0
+    assert ta == tb
+    assert np.array_equal(interior(A), interior(B))
+
+
+@pytest.mark.parametrize("rec,riemann", [("minmod", "hll"), ("ppm", "hllc"), ("wenoz", "hll")])
+def test_fused_xy_all_methods(rec, riemann):
+    progress(f"test_fused_xy_all_methods {rec} {riemann}")
+    P = state3(70, 20, 16, seed=22)
+    A, _, _ = _run(P, True, 2, rec=rec, riemann=riemann, integrator="rk2", ref=True,
+                   bc=["periodic"] * 6)
+    B, _, _ = _run(P, False, 2, rec=rec, riemann=riemann, integrator="rk2", ref=True,
+                   bc=["periodic"] * 6)
+    assert np.array_equal(interior(A), interior(B))

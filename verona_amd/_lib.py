@@ -65,6 +65,7 @@ SIGNATURES = {
     "vrn_timing_report": (_I, [_P, C.c_char_p, _I, _I64P, _DP, _I, C.POINTER(_I)]),
     "vrn_set_partition": (_I, [_P, _I, _I, _I]),
     "vrn_get_partition": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "vrn_set_xy_fusion": (_I, [_P, _I]),
 }
 
 

@@ -44,6 +44,18 @@ __device__ __forceinline__ double shift_up1(double v, double fill)
     const int hi = dpp_shr1((int)(b >> 32), (int)(f >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// Value of lane l+1, and `fill` in lane 63: DPP wave_shl:1, bound_ctrl off.
+__device__ __forceinline__ int dpp_shl1(int x, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double shift_down1(double v, double fill)
+{
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = dpp_shl1((int)(b & 0xffffffffll), (int)(f & 0xffffffffll));
+    const int hi = dpp_shl1((int)(b >> 32), (int)(f >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 // Value of lane 63 in every lane (v_readlane into a scalar register).
 __device__ __forceinline__ float lane63(float v)
 {
@@ -265,18 +277,67 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
         if (a.do_rate) rate_reduce(a.st, rate);
 }
 
+// ------------------------------------------------------------ x faces at wave boundaries
+// For the fused x-y march: one thread per (boundary b, interior row j, k)
+// solves the x face between cells xf - 1 and xf, xf = min(3 + 64 b, NX - 3),
+// with the x sweep's arithmetic (the reconstructions of both cells, their
+// floors, face_flux), so the march's fluxes at its 64-column wave edges are
+// the faces the x sweep would solve there.
+template <int NV, class R, int REC, int RIEM>
+__global__ __launch_bounds__(256) void k_xfaces(SweepArgs a)
+{
+    const Geo G = a.geo;
+    const int Nyi = G.NY - 6, Nzi = G.NZ - 6, nbf = a.nbf;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nbf * Nyi * Nzi) return;
+    const int b = (int)(t % nbf);
+    const long r = t / nbf;
+    const int j = 3 + (int)(r % Nyi), k = 3 + (int)(r / Nyi);
+    const int xf = min(3 + 64 * b, G.NX - 3);
+    const long rb = gidx(G, 0, j, k);
+    const long ps = G.ps;
+    const R g = (R)a.g, fl = (R)a.floor_;
+    R up[NV], lo[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double *q = a.P + v * ps + rb + xf;
+        R t0, t1;
+        recon<REC>((R)q[-3], (R)q[-2], (R)q[-1], (R)q[0], (R)q[1], t0, up[v]);
+        recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], t1);
+    }
+    lo[0] = floor_max(fl, lo[0]);
+    lo[1] = floor_max(fl, lo[1]);
+    up[0] = floor_max(fl, up[0]);
+    up[1] = floor_max(fl, up[1]);
+    R Ff[NV];
+    face_flux<NV, 0, RIEM>(up, lo, g, Ff);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        a.bface[(((long)v * G.NZ + k) * G.NY + j) * nbf + b] = (double)Ff[v];
+}
+
 // ------------------------------------------------------------ y / z sweeps
 // One lane per interior (x, o) column, marching along DIM over the segment
 // [m0, m1) of interior cells.  The 5-cell reconstruction window of each lane
 // lives in LDS as a ring of 5 slots laid out [slot][var][lane] (each lane
 // touches only its own words: conflict-free and barrier-free), which keeps
 // it out of the VGPR budget of the Riemann solve and con2prim.
-template <int NV, int DIM, class R, int REC, int RIEM, bool LAST>
+template <int NV, int DIM, class R, int REC, int RIEM, bool LAST, bool XY = false>
 __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 {
     // lanes per block: 256 in 3D (four o-rows of 64 columns), 64 in 2D
     constexpr int WS = NV == 5 ? 256 : 64;
-    __shared__ R win[5 * NV * WS];
+    // XY (3D y march only): the march also does the x direction of each row
+    // it updates.  Each wave's ring row then carries two halo columns on
+    // either side of its 64 (x0-2, x0-1 | x0+64, x0+65) for the x stencils.
+    static_assert(!XY || (DIM == 1 && NV == 5 && !LAST), "x-y fusion is the 3D y march");
+    constexpr int HW = XY ? 2 : 0;
+    constexpr int RW = 64 + 2 * HW;
+    constexpr int WSR = (WS / 64) * RW;  // ring stride per (slot, variable)
+    __shared__ R win[5 * NV * WSR];
+    // XY: the x faces at each wave's two edges for the rows in the window,
+    // [slot][wave][side][var] (side 0: left edge, lane 0; 1: right edge, lane 63)
+    __shared__ double bring[XY ? 5 * (WS / 64) * 2 * NV : 1];
     // WENO-Z: the curvature term of each cell (wz_curv) is computed once and
     // carried: cell c needs the terms of cells c-1, c and c+1, and the term of
     // cell x lives in slot x & 1 of a two-slot ring [slot][var][lane].
@@ -293,7 +354,9 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     // cross-lane reductions and masks the inactive ones.
     if (o >= No - 3 || m0 >= Nd - 3) return;
     const bool act = x < G.NX - 3;
-    const int xx = act ? x : 3;
+    // XY: every lane holds its own column (inactive lanes a ghost or the
+    // last one), since the x stencils of the active lanes reach into them
+    const int xx = XY ? min(x, G.NX - 1) : (act ? x : 3);
     const int m1 = min(m0 + a.seg_len, Nd - 3);
     const long sd = DIM == 1 ? (long)G.px : (long)G.px * G.NY;
     const long cb = DIM == 1 ? gidx(G, xx, 0, NV == 5 ? o : 0) : gidx(G, xx, o, 0);
@@ -301,8 +364,10 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     const R g = (R)a.g, fl = (R)a.floor_;
     const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
     const double *__restrict__ P = a.P;
-    const double *Ui = a.Uacc;
-    R *wl = win + tid;  // slot s, variable v at wl[(s * NV + v) * WS]
+    // the x-y march starts from the stage's base state (there is no x sweep)
+    const double *Ui = XY ? a.Ub : a.Uacc;
+    const int lane = tid & 63;
+    R *wl = win + (tid >> 6) * RW + HW + lane;  // slot s, variable v at wl[(s * NV + v) * WSR]
     R *kl = curv + tid;  // curvature term of cell x, variable v at kl[((x & 1) * NV + v) * WS]
 
     // ring: cell m lives in slot (m - (m0 - 3)) mod 5
@@ -311,16 +376,51 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     for (int v = 0; v < NV; v++) {
         const double *q = P + v * ps + cb;
 #pragma unroll
-        for (int k = 0; k < 4; k++) wl[(k * NV + v) * WS] = (R)q[(m0 - 3 + k) * sd];
+        for (int k = 0; k < 4; k++) wl[(k * NV + v) * WSR] = (R)q[(m0 - 3 + k) * sd];
         nx[v] = q[(m0 + 1) * sd];
         if constexpr (CARRY) {
             // cells m0-2 and m0-1, the neighbours of the first window's centre
             kl[(((m0 - 2) & 1) * NV + v) * WS] =
-                wz_curv(wl[(0 * NV + v) * WS], wl[(1 * NV + v) * WS], wl[(2 * NV + v) * WS]);
+                wz_curv(wl[(0 * NV + v) * WSR], wl[(1 * NV + v) * WSR], wl[(2 * NV + v) * WSR]);
             kl[(((m0 - 1) & 1) * NV + v) * WS] =
-                wz_curv(wl[(1 * NV + v) * WS], wl[(2 * NV + v) * WS], wl[(3 * NV + v) * WS]);
+                wz_curv(wl[(1 * NV + v) * WSR], wl[(2 * NV + v) * WSR], wl[(3 * NV + v) * WSR]);
         }
     }
+    // XY side data of each ring row, one value per lane so that the
+    // prefetch costs one register: lanes 0..19 carry the halo columns
+    // (variable lane / 4; lane % 4 = 0, 1, 2, 3 -> x0-2, x0-1, x0+64, x0+65,
+    // clamped to the grid), lanes 20..29 the edge faces from k_xfaces (side
+    // (lane - 20) / 5, variable (lane - 20) % 5; interior rows only).
+    const int x0 = 3 + blockIdx.x * 64;
+    const int wv = tid >> 6;
+    const bool hlane = XY && lane < 20, blane = XY && lane >= 20 && lane < 30;
+    // (lanes outside each group get in-range indices they never use)
+    const int lb = min(max(lane - 20, 0), 9);
+    const int hv = min(lane >> 2, NV - 1), hh = lane & 3, bs = lb / 5, bv = lb % 5;
+    const int hcol = min(max(x0 + (hh < 2 ? hh - 2 : 62 + hh), 0), G.NX - 1);
+    // ring word of this lane's halo value (slot 0); slot s adds s * NV * WSR
+    R *hl = win + hv * WSR + wv * RW + (hh < 2 ? hh : 64 + hh);
+    const double *hsrc = P + (long)hv * ps + gidx(G, hcol, 0, o);
+    // this lane's edge face of row j: bsrc[j * nbf]; its bring word (slot 0)
+    const double *bsrc = XY ? a.bface + ((long)bv * G.NZ + o) * G.NY * a.nbf + blockIdx.x + bs : nullptr;
+    double *bl = bring + (wv * 2 + bs) * NV + bv;
+    constexpr int BSL = (WS / 64) * 2 * NV;  // bring stride per slot
+    double xv = 0.0;
+    if constexpr (XY) {
+        if (hlane) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) hl[k * NV * WSR] = (R)hsrc[(m0 - 3 + k) * sd];
+            xv = hsrc[(m0 + 1) * sd];
+        } else if (blane) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int j = m0 - 3 + k;
+                bl[k * BSL] = (j >= 3 && j < G.NY - 3) ? bsrc[(long)j * a.nbf] : 0.0;
+            }
+            xv = (m0 + 1 < G.NY - 3) ? bsrc[(long)(m0 + 1) * a.nbf] : 0.0;
+        }
+    }
+    const double cdtx = XY ? ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_x : 0.0;
     R upp[NV];
     double Fp[NV];
 #pragma unroll
@@ -334,10 +434,23 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         const int s1 = s0 == 4 ? 0 : s0 + 1, s2 = s1 == 4 ? 0 : s1 + 1;
         const int s3 = s2 == 4 ? 0 : s2 + 1, s4 = s3 == 4 ? 0 : s3 + 1;
 #pragma unroll
-        for (int v = 0; v < NV; v++) wl[(s4 * NV + v) * WS] = (R)nx[v];
+        for (int v = 0; v < NV; v++) wl[(s4 * NV + v) * WSR] = (R)nx[v];
+        if constexpr (XY) {
+            if (hlane) hl[s4 * NV * WSR] = (R)xv;
+            else if (blane) bl[s4 * BSL] = xv;
+            // the x stencils read other lanes' ring words (written by this
+            // wave); keep the compiler from moving the reads above the writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         if (c + 3 < Nd) {
 #pragma unroll
             for (int v = 0; v < NV; v++) nx[v] = ld_once(&P[v * ps + cb + (c + 3) * sd]);
+            if constexpr (XY) {
+                if (hlane) xv = ld_once(&hsrc[(c + 3) * sd]);
+                else if (blane) xv = (c + 3 < G.NY - 3) ? bsrc[(long)(c + 3) * a.nbf] : 0.0;
+            }
         }
         // conserved state of cell c-1 (updated at the end of this iteration):
         // issued now, consumed after the reconstruction and the Riemann solve
@@ -353,14 +466,14 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             if constexpr (CARRY) {
-                const R q2 = wl[(s2 * NV + v) * WS], q3 = wl[(s3 * NV + v) * WS], q4 = wl[(s4 * NV + v) * WS];
+                const R q2 = wl[(s2 * NV + v) * WSR], q3 = wl[(s3 * NV + v) * WSR], q4 = wl[(s4 * NV + v) * WSR];
                 R *kp = kl + (((c + 1) & 1) * NV + v) * WS;  // cell c-1's term, then cell c+1's
                 const R km = *kp, k0 = kl[((c & 1) * NV + v) * WS], kn = wz_curv(q2, q3, q4);
                 *kp = kn;
-                wenoz_k(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], q2, q3, q4, km, k0, kn, lo[v], up[v]);
+                wenoz_k(wl[(s0 * NV + v) * WSR], wl[(s1 * NV + v) * WSR], q2, q3, q4, km, k0, kn, lo[v], up[v]);
             } else {
-                recon<REC>(wl[(s0 * NV + v) * WS], wl[(s1 * NV + v) * WS], wl[(s2 * NV + v) * WS],
-                           wl[(s3 * NV + v) * WS], wl[(s4 * NV + v) * WS], lo[v], up[v]);
+                recon<REC>(wl[(s0 * NV + v) * WSR], wl[(s1 * NV + v) * WSR], wl[(s2 * NV + v) * WSR],
+                           wl[(s3 * NV + v) * WSR], wl[(s4 * NV + v) * WSR], lo[v], up[v]);
             }
         }
         lo[0] = floor_max(fl, lo[0]);
@@ -374,6 +487,41 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
             for (int v = 0; v < NV; v++) F[v] = (double)Ff[v];
             if (upd) {
+                if constexpr (XY) {
+                    // x direction of row c-1, as the x sweep does it: each
+                    // lane reconstructs its cell (stencil from the ring row),
+                    // solves its left face with its left neighbour's upper
+                    // state (DPP shift) and takes its right face from its
+                    // right neighbour; the wave's edge faces come from
+                    // k_xfaces.  Uniform control flow: the shifts read every
+                    // lane of the wave.
+                    R lox[NV], upx[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        const R *rw = wl + (s1 * NV + v) * WSR;
+                        recon<REC>(rw[-2], rw[-1], rw[0], rw[1], rw[2], lox[v], upx[v]);
+                    }
+                    lox[0] = floor_max(fl, lox[0]);
+                    lox[1] = floor_max(fl, lox[1]);
+                    upx[0] = floor_max(fl, upx[0]);
+                    upx[1] = floor_max(fl, upx[1]);
+                    R uplx[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) uplx[v] = shift_up1(upx[v], upx[v]);
+                    R Fxf[NV];
+                    face_flux<NV, 0, RIEM>(uplx, lox, g, Fxf);
+                    // the wave's edge faces of row c-1 (k_xfaces): lane 0
+                    // reads the left one, every other lane the right one
+                    const double *be = bring + s1 * BSL + (wv * 2 + (lane == 0 ? 0 : 1)) * NV;
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        const double bfv = be[v];
+                        const double fxl = lane == 0 ? bfv : (double)Fxf[v];
+                        const double fxs = shift_down1(fxl, fxl);
+                        const double fxr = lane == 63 ? bfv : fxs;
+                        ub[v] = ub[v] - cdtx * (fxr - fxl);
+                    }
+                }
                 double acc[NV];
 #pragma unroll
                 for (int v = 0; v < NV; v++) acc[v] = ub[v] - cdt * (F[v] - Fp[v]);
@@ -388,7 +536,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
                         if constexpr (std::is_same<R, double>::value)
-                            guess[v] = wl[(s1 * NV + v) * WS];
+                            guess[v] = wl[(s1 * NV + v) * WSR];
                         else
                             guess[v] = P[v * ps + cu];
                     }
@@ -478,7 +626,33 @@ template <int NV, class R, int REC, int RIEM>
 void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (*timer)(void *, int, int), void *tctx)
 {
     const Geo &G = a.geo;
-    {
+    const int nxb = (G.NX - 6 + 63) / 64;
+    // 3D fast FP64 form: the x direction runs inside the y march (k_xfaces
+    // first solves the faces at the marches' 64-column wave edges), which
+    // saves the x sweep's write and the y sweep's re-read of the
+    // accumulated U and one read of P per stage; the same fluxes and
+    // updates in the same order as the x and y sweeps.
+#if defined(VRN_FAST)
+    const bool fxy = NV == 5 && std::is_same<R, double>::value && a.fuse_xy && a.bface != nullptr;
+#else
+    const bool fxy = false;
+#endif
+    if (fxy) {
+        if constexpr (NV == 5 && std::is_same<R, double>::value) {
+            const long nf = (long)a.nbf * (G.NY - 6) * (G.NZ - 6);
+            timer(tctx, 10, 0);
+            hipLaunchKernelGGL((k_xfaces<5, R, REC, RIEM>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, a);
+            timer(tctx, 10, 1);
+            SweepArgs ay = a;
+            ay.dx_dim = dxs[1];
+            ay.dx_x = dxs[0];
+            int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
+            timer(tctx, 11, 0);
+            hipLaunchKernelGGL((k_march<5, 1, R, REC, RIEM, false, true>), dim3(nxb, (G.NZ - 6 + 3) / 4, nseg),
+                               dim3(256), 0, s, ay);
+            timer(tctx, 11, 1);
+        }
+    } else {
         SweepArgs ax = a;
         ax.dx_dim = dxs[0];
         int nrows = (G.NY - 6) * (NV == 5 ? G.NZ - 6 : 1);
@@ -487,9 +661,8 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
         hipLaunchKernelGGL((k_xsweep<NV, R, REC, RIEM, false>), dim3((waves + 3) / 4), dim3(256), 0, s, ax);
         timer(tctx, 0, 1);
     }
-    const int nxb = (G.NX - 6 + 63) / 64;
     if constexpr (NV == 5) {
-        {
+        if (!fxy) {
             SweepArgs ay = a;
             ay.dx_dim = dxs[1];
             int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;

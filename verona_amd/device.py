@@ -143,6 +143,11 @@ class DeviceGrid:
         check(self.lib.vrn_set_partition(self.h, int(rows_per_wave), int(xseg), int(seg_len)),
               "vrn_set_partition")
 
+    def set_xy_fusion(self, on=True):
+        """3D fast FP64 form: x direction inside the y march (default) or as
+        its own sweep; the same arithmetic either way."""
+        check(self.lib.vrn_set_xy_fusion(self.h, int(bool(on))), "vrn_set_xy_fusion")
+
     def partition(self):
         r, x, s = C.c_int(), C.c_int(), C.c_int()
         check(self.lib.vrn_get_partition(self.h, C.byref(r), C.byref(x), C.byref(s)), "vrn_get_partition")
