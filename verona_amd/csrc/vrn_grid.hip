@@ -648,10 +648,11 @@ int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
     c->stg = p;
     e = hipMalloc(&c->rescue, sizeof(int) * (size_t)cells);
     if (e == hipSuccess && ndim == 3) {
-        // boundary x faces of the fused x-y march: one per 64-column group
-        // of a row and one at its end, 5 variables (42.6 MB at 384^3)
+        // x faces at the wave edges of the fused x-y march: five around each
+        // 64-column group boundary of a row and its end, 5 variables
+        // (213 MB at 384^3)
         c->nbf = (nx + 63) / 64 + 1;
-        e = hipMalloc(&c->bface, sizeof(double) * 5 * (size_t)c->nbf * c->G.NY * c->G.NZ);
+        e = hipMalloc(&c->bface, sizeof(double) * 25 * (size_t)c->nbf * c->G.NY * c->G.NZ);
     }
     if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(DevState));
     if (e == hipSuccess) e = hipHostMalloc(&c->st_host, sizeof(DevState), hipHostMallocDefault);

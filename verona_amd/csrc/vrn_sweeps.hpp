@@ -278,22 +278,35 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
 }
 
 // ------------------------------------------------------------ x faces at wave boundaries
-// For the fused x-y march: one thread per (boundary b, interior row j, k)
-// solves the x face between cells xf - 1 and xf, xf = min(3 + 64 b, NX - 3),
-// with the x sweep's arithmetic (the reconstructions of both cells, their
-// floors, face_flux), so the march's fluxes at its 64-column wave edges are
-// the faces the x sweep would solve there.
+// For the fused x-y march: the five x faces around each 64-column wave edge
+// xb = min(3 + 64 b, NX - 3) of every interior row, face f (0..4) between
+// cells xf - 1 and xf with xf = xb - 2 + f, solved with the x sweep's
+// arithmetic (the reconstructions of both cells, their floors, face_flux).
+// The march's three lanes at either end of a wave, whose stencils would reach
+// past its 64 columns, take these faces instead: lanes 0..2 faces 2..4 of
+// their wave's left edge, lanes 62, 63 faces 0..2 of its right edge.  Faces
+// without a full stencil in the grid (xf < 3 or xf > NX - 3, never used) are
+// stored as 0.  One thread per face.
 template <int NV, class R, int REC, int RIEM>
 __global__ __launch_bounds__(256) void k_xfaces(SweepArgs a)
 {
     const Geo G = a.geo;
     const int Nyi = G.NY - 6, Nzi = G.NZ - 6, nbf = a.nbf;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nbf * Nyi * Nzi) return;
-    const int b = (int)(t % nbf);
-    const long r = t / nbf;
+    if (t >= 5L * nbf * Nyi * Nzi) return;
+    const int f = (int)(t % 5);
+    const int b = (int)((t / 5) % nbf);
+    const long r = t / (5L * nbf);
     const int j = 3 + (int)(r % Nyi), k = 3 + (int)(r / Nyi);
-    const int xf = min(3 + 64 * b, G.NX - 3);
+    const int xf = min(3 + 64 * b, G.NX - 3) - 2 + f;
+    double *out = a.bface + ((long)j * nbf + b) * 5 + f;
+    const long vs = (long)G.NZ * G.NY * nbf * 5;  // bface stride per variable (k folded in below)
+    out += (long)k * G.NY * nbf * 5;
+    if (xf < 3 || xf > G.NX - 3) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) out[v * vs] = 0.0;
+        return;
+    }
     const long rb = gidx(G, 0, j, k);
     const long ps = G.ps;
     const R g = (R)a.g, fl = (R)a.floor_;
@@ -312,8 +325,7 @@ __global__ __launch_bounds__(256) void k_xfaces(SweepArgs a)
     R Ff[NV];
     face_flux<NV, 0, RIEM>(up, lo, g, Ff);
 #pragma unroll
-    for (int v = 0; v < NV; v++)
-        a.bface[(((long)v * G.NZ + k) * G.NY + j) * nbf + b] = (double)Ff[v];
+    for (int v = 0; v < NV; v++) out[v * vs] = (double)Ff[v];
 }
 
 // ------------------------------------------------------------ y / z sweeps
@@ -328,16 +340,14 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     // lanes per block: 256 in 3D (four o-rows of 64 columns), 64 in 2D
     constexpr int WS = NV == 5 ? 256 : 64;
     // XY (3D y march only): the march also does the x direction of each row
-    // it updates.  Each wave's ring row then carries two halo columns on
-    // either side of its 64 (x0-2, x0-1 | x0+64, x0+65) for the x stencils.
+    // it updates, with the ring row of its 64 columns as the x stencils; the
+    // six faces at a wave's ends (three each side, k_xfaces) sit in a second
+    // ring [slot][wave][face][var].
     static_assert(!XY || (DIM == 1 && NV == 5 && !LAST), "x-y fusion is the 3D y march");
-    constexpr int HW = XY ? 2 : 0;
-    constexpr int RW = 64 + 2 * HW;
-    constexpr int WSR = (WS / 64) * RW;  // ring stride per (slot, variable)
+    constexpr int WSR = WS;  // ring stride per (slot, variable)
     __shared__ R win[5 * NV * WSR];
-    // XY: the x faces at each wave's two edges for the rows in the window,
-    // [slot][wave][side][var] (side 0: left edge, lane 0; 1: right edge, lane 63)
-    __shared__ double bring[XY ? 5 * (WS / 64) * 2 * NV : 1];
+    constexpr int BSL = (WS / 64) * 6 * NV;  // edge-face ring stride per slot
+    __shared__ double bring[XY ? 5 * BSL : 1];
     // WENO-Z: the curvature term of each cell (wz_curv) is computed once and
     // carried: cell c needs the terms of cells c-1, c and c+1, and the term of
     // cell x lives in slot x & 1 of a two-slot ring [slot][var][lane].
@@ -367,7 +377,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     // the x-y march starts from the stage's base state (there is no x sweep)
     const double *Ui = XY ? a.Ub : a.Uacc;
     const int lane = tid & 63;
-    R *wl = win + (tid >> 6) * RW + HW + lane;  // slot s, variable v at wl[(s * NV + v) * WSR]
+    R *wl = win + tid;  // slot s, variable v at wl[(s * NV + v) * WSR]
     R *kl = curv + tid;  // curvature term of cell x, variable v at kl[((x & 1) * NV + v) * WS]
 
     // ring: cell m lives in slot (m - (m0 - 3)) mod 5
@@ -386,40 +396,38 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                 wz_curv(wl[(1 * NV + v) * WSR], wl[(2 * NV + v) * WSR], wl[(3 * NV + v) * WSR]);
         }
     }
-    // XY side data of each ring row, one value per lane so that the
-    // prefetch costs one register: lanes 0..19 carry the halo columns
-    // (variable lane / 4; lane % 4 = 0, 1, 2, 3 -> x0-2, x0-1, x0+64, x0+65,
-    // clamped to the grid), lanes 20..29 the edge faces from k_xfaces (side
-    // (lane - 20) / 5, variable (lane - 20) % 5; interior rows only).
-    const int x0 = 3 + blockIdx.x * 64;
+    // XY edge faces of each ring row, one value per lane so that the
+    // prefetch costs one register: lane L < 30 carries variable L % 5 of
+    // edge face L / 5 = 0, 1, 2 (faces 2, 3, 4 of the wave's left edge
+    // b = blockIdx.x) or 3, 4, 5 (faces 0, 1, 2 of its right edge b + 1);
+    // interior rows only.  Lanes 0..2 and 62, 63 use them (the x part).
     const int wv = tid >> 6;
-    const bool hlane = XY && lane < 20, blane = XY && lane >= 20 && lane < 30;
-    // (lanes outside each group get in-range indices they never use)
-    const int lb = min(max(lane - 20, 0), 9);
-    const int hv = min(lane >> 2, NV - 1), hh = lane & 3, bs = lb / 5, bv = lb % 5;
-    const int hcol = min(max(x0 + (hh < 2 ? hh - 2 : 62 + hh), 0), G.NX - 1);
-    // ring word of this lane's halo value (slot 0); slot s adds s * NV * WSR
-    R *hl = win + hv * WSR + wv * RW + (hh < 2 ? hh : 64 + hh);
-    const double *hsrc = P + (long)hv * ps + gidx(G, hcol, 0, o);
-    // this lane's edge face of row j: bsrc[j * nbf]; its bring word (slot 0)
-    const double *bsrc = XY ? a.bface + ((long)bv * G.NZ + o) * G.NY * a.nbf + blockIdx.x + bs : nullptr;
-    double *bl = bring + (wv * 2 + bs) * NV + bv;
-    constexpr int BSL = (WS / 64) * 2 * NV;  // bring stride per slot
+    const int lb = min(lane, 29);  // lanes past 29 get in-range indices they never use
+    const int fe = lb / 5, fv = lb % 5;
+    const bool blane = XY && lane < 30;
+    // + j * nbf * 5 for row j
+    const double *bsrc = XY ? a.bface + ((long)fv * G.NZ + o) * G.NY * a.nbf * 5 +
+                                  (blockIdx.x + (fe < 3 ? 0 : 1)) * 5 + (fe < 3 ? fe + 2 : fe - 3)
+                            : nullptr;
+    double *bl = bring + (wv * 6 + fe) * NV + fv;
     double xv = 0.0;
     if constexpr (XY) {
-        if (hlane) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) hl[k * NV * WSR] = (R)hsrc[(m0 - 3 + k) * sd];
-            xv = hsrc[(m0 + 1) * sd];
-        } else if (blane) {
+        if (blane) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int j = m0 - 3 + k;
-                bl[k * BSL] = (j >= 3 && j < G.NY - 3) ? bsrc[(long)j * a.nbf] : 0.0;
+                bl[k * BSL] = (j >= 3 && j < G.NY - 3) ? bsrc[(long)j * a.nbf * 5] : 0.0;
             }
-            xv = (m0 + 1 < G.NY - 3) ? bsrc[(long)(m0 + 1) * a.nbf] : 0.0;
+            xv = (m0 + 1 < G.NY - 3) ? bsrc[(long)(m0 + 1) * a.nbf * 5] : 0.0;
         }
     }
+    // XY: the edge face that is this lane's left face (-1: solved in the
+    // wave): lanes 0..2 -> edge faces 0..2; lanes 62, 63 -> the right edge's
+    // face at their column (x0 + lane relative to the edge xr = min(x0 + 64,
+    // NX - 3)); lane 63 also takes its right face, edge face 5
+    const int x0 = 3 + blockIdx.x * 64;
+    const int xr = min(x0 + 64, G.NX - 3);
+    const int lfe = lane < 3 ? lane : (lane >= 62 ? 3 + min(max(x0 + lane - xr + 2, 0), 2) : -1);
     const double cdtx = XY ? ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_x : 0.0;
     R upp[NV];
     double Fp[NV];
@@ -436,8 +444,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
         for (int v = 0; v < NV; v++) wl[(s4 * NV + v) * WSR] = (R)nx[v];
         if constexpr (XY) {
-            if (hlane) hl[s4 * NV * WSR] = (R)xv;
-            else if (blane) bl[s4 * BSL] = xv;
+            if (blane) bl[s4 * BSL] = xv;
             // the x stencils read other lanes' ring words (written by this
             // wave); keep the compiler from moving the reads above the writes
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -448,8 +455,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
             for (int v = 0; v < NV; v++) nx[v] = ld_once(&P[v * ps + cb + (c + 3) * sd]);
             if constexpr (XY) {
-                if (hlane) xv = ld_once(&hsrc[(c + 3) * sd]);
-                else if (blane) xv = (c + 3 < G.NY - 3) ? bsrc[(long)(c + 3) * a.nbf] : 0.0;
+                if (blane) xv = (c + 3 < G.NY - 3) ? bsrc[(long)(c + 3) * a.nbf * 5] : 0.0;
             }
         }
         // conserved state of cell c-1 (updated at the end of this iteration):
@@ -492,9 +498,9 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                     // lane reconstructs its cell (stencil from the ring row),
                     // solves its left face with its left neighbour's upper
                     // state (DPP shift) and takes its right face from its
-                    // right neighbour; the wave's edge faces come from
-                    // k_xfaces.  Uniform control flow: the shifts read every
-                    // lane of the wave.
+                    // right neighbour; the faces whose stencils leave the
+                    // wave's 64 columns come from k_xfaces.  Uniform control
+                    // flow: the shifts read every lane of the wave.
                     R lox[NV], upx[NV];
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
@@ -510,15 +516,15 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                     for (int v = 0; v < NV; v++) uplx[v] = shift_up1(upx[v], upx[v]);
                     R Fxf[NV];
                     face_flux<NV, 0, RIEM>(uplx, lox, g, Fxf);
-                    // the wave's edge faces of row c-1 (k_xfaces): lane 0
-                    // reads the left one, every other lane the right one
-                    const double *be = bring + s1 * BSL + (wv * 2 + (lane == 0 ? 0 : 1)) * NV;
+                    // edge faces of row c-1 (k_xfaces): each lane reads its
+                    // own (lfe) and edge face 5 (lane 63's right face)
+                    const double *be = bring + s1 * BSL + wv * 6 * NV;
+                    const int lf = lfe < 0 ? 0 : lfe;
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
-                        const double bfv = be[v];
-                        const double fxl = lane == 0 ? bfv : (double)Fxf[v];
+                        const double fxl = lfe >= 0 ? be[lf * NV + v] : (double)Fxf[v];
                         const double fxs = shift_down1(fxl, fxl);
-                        const double fxr = lane == 63 ? bfv : fxs;
+                        const double fxr = lane == 63 ? be[5 * NV + v] : fxs;
                         ub[v] = ub[v] - cdtx * (fxr - fxl);
                     }
                 }
@@ -639,7 +645,7 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
 #endif
     if (fxy) {
         if constexpr (NV == 5 && std::is_same<R, double>::value) {
-            const long nf = (long)a.nbf * (G.NY - 6) * (G.NZ - 6);
+            const long nf = 5L * a.nbf * (G.NY - 6) * (G.NZ - 6);
             timer(tctx, 10, 0);
             hipLaunchKernelGGL((k_xfaces<5, R, REC, RIEM>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, a);
             timer(tctx, 10, 1);
