@@ -388,8 +388,8 @@ int fail(int code, const char *fmt, ...)
     } while (0)
 
 const char *kKernelNames[] = {"xsweep", "ysweep", "zsweep_c2p", "finish_2d", "rescue", "ghost",
-                              "cfl",    "fluxes", "update",     "c2p",       "xfaces", "xysweep"};
-enum { KN_X, KN_Y, KN_ZC, KN_F2D, KN_RESCUE, KN_GHOST, KN_CFL, KN_FLUX, KN_UPD, KN_C2P, KN_XF, KN_XY, KN_COUNT };
+                              "cfl",    "fluxes", "update",     "c2p",       "xysweep"};
+enum { KN_X, KN_Y, KN_ZC, KN_F2D, KN_RESCUE, KN_GHOST, KN_CFL, KN_FLUX, KN_UPD, KN_C2P, KN_XY, KN_COUNT };
 
 }  // namespace
 
@@ -399,9 +399,7 @@ struct vrn_ctx {
     double gamma = 4.0 / 3.0;
     double *buf = nullptr;   // P, Pn, U, Uh and the staging buffer
     double *fbuf = nullptr;  // flux slots, allocated by the first per-kernel call that uses them
-    double *bface = nullptr;  // 3D: x faces at the wave boundaries of every row (fused x-y march)
-    int nbf = 0;
-    int fuse_xy = 1;
+    int fuse_xy = 1;  // 3D fast FP64 form: x direction inside the y march
     double *P = nullptr, *Pn = nullptr, *U = nullptr, *Uh = nullptr, *F[3] = {nullptr, nullptr, nullptr};
     double *stg = nullptr;
     int *rescue = nullptr;
@@ -647,13 +645,6 @@ int vrn_create(int ndim, int nx, int ny, int nz, double gamma, vrn_ctx **out)
     c->Uh = p; p += arr;
     c->stg = p;
     e = hipMalloc(&c->rescue, sizeof(int) * (size_t)cells);
-    if (e == hipSuccess && ndim == 3) {
-        // x faces at the wave edges of the fused x-y march: five around each
-        // 64-column group boundary of a row and its end, 5 variables
-        // (213 MB at 384^3)
-        c->nbf = (nx + 63) / 64 + 1;
-        e = hipMalloc(&c->bface, sizeof(double) * 25 * (size_t)c->nbf * c->G.NY * c->G.NZ);
-    }
     if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(DevState));
     if (e == hipSuccess) e = hipHostMalloc(&c->st_host, sizeof(DevState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(c->buf, 0, total * sizeof(double), c->s);
@@ -681,7 +672,6 @@ int vrn_destroy(vrn_ctx *c)
     }
     if (c->buf) (void)hipFree(c->buf);
     if (c->fbuf) (void)hipFree(c->fbuf);
-    if (c->bface) (void)hipFree(c->bface);
     if (c->rescue) (void)hipFree(c->rescue);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
@@ -1033,8 +1023,6 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     a.seg_len = c->seg_len;
     a.rows_per_wave = c->rows_per_wave;
     a.xseg = c->xseg;
-    a.bface = c->bface;
-    a.nbf = c->nbf;
     a.fuse_xy = c->fuse_xy;
     a.dx_x = p->dx;
     const int nst = p->integrator == VRN_INT_RK3 ? 3 : 2;

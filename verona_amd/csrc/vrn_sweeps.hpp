@@ -277,55 +277,52 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
         if (a.do_rate) rate_reduce(a.st, rate);
 }
 
-// ------------------------------------------------------------ x faces at wave boundaries
-// For the fused x-y march: the five x faces around each 64-column wave edge
-// xb = min(3 + 64 b, NX - 3) of every interior row, face f (0..4) between
-// cells xf - 1 and xf with xf = xb - 2 + f, solved with the x sweep's
-// arithmetic (the reconstructions of both cells, their floors, face_flux).
-// The march's three lanes at either end of a wave, whose stencils would reach
-// past its 64 columns, take these faces instead: lanes 0..2 faces 2..4 of
-// their wave's left edge, lanes 62, 63 faces 0..2 of its right edge.  Faces
-// without a full stencil in the grid (xf < 3 or xf > NX - 3, never used) are
-// stored as 0.  One thread per face.
+// ------------------------------------------------------------ x-y march: wave-edge faces
+// The six x faces of rows r0 .. r1-1 at the ends of one wave's 64 columns
+// [x0, x0 + 64) whose stencils leave those columns: faces between cells
+// m - 1 and m for m = x0, x0 + 1, x0 + 2 (edge faces 0..2) and m = xr - 2,
+// xr - 1, xr with xr = min(x0 + 64, NX - 3) (edge faces 3..5).  Lane L
+// reconstructs cell x0 - 1 + i (i = L % 8 < 4) or xr - 3 + (i - 4) of row
+// r0 + L / 8, as the x sweep does (both faces, floors), and lanes with i != 0, 4
+// solve the face with their left neighbour's upper state (DPP shift):
+// exactly the x sweep's faces.  Faces of rows outside [r0, r1) or whose
+// stencil leaves the grid are not needed and not written.  Out: ebw[(row -
+// rb) & 7 row stride ebs][face][var].
 template <int NV, class R, int REC, int RIEM>
-__global__ __launch_bounds__(256) void k_xfaces(SweepArgs a)
+__device__ __forceinline__ void edge_faces(const SweepArgs &a, const Geo &G, double *ebw, int ebs, int lane, int x0,
+                                           int xr, int o, int r0, int r1, int rb)
 {
-    const Geo G = a.geo;
-    const int Nyi = G.NY - 6, Nzi = G.NZ - 6, nbf = a.nbf;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 5L * nbf * Nyi * Nzi) return;
-    const int f = (int)(t % 5);
-    const int b = (int)((t / 5) % nbf);
-    const long r = t / (5L * nbf);
-    const int j = 3 + (int)(r % Nyi), k = 3 + (int)(r / Nyi);
-    const int xf = min(3 + 64 * b, G.NX - 3) - 2 + f;
-    double *out = a.bface + ((long)j * nbf + b) * 5 + f;
-    const long vs = (long)G.NZ * G.NY * nbf * 5;  // bface stride per variable (k folded in below)
-    out += (long)k * G.NY * nbf * 5;
-    if (xf < 3 || xf > G.NX - 3) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) out[v * vs] = 0.0;
-        return;
-    }
-    const long rb = gidx(G, 0, j, k);
+    const int i = lane & 7, r = r0 + (lane >> 3);
+    const int cell = i < 4 ? x0 - 1 + i : xr - 7 + i;
+    const bool ok = r < r1 && cell >= 2 && cell <= G.NX - 3;
+    const int cc = ok ? cell : 3;
+    const int rr = ok ? r : 3;
+    const long rb0 = gidx(G, cc, rr, o);
     const long ps = G.ps;
     const R g = (R)a.g, fl = (R)a.floor_;
-    R up[NV], lo[NV];
+    R lo[NV], up[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-        const double *q = a.P + v * ps + rb + xf;
-        R t0, t1;
-        recon<REC>((R)q[-3], (R)q[-2], (R)q[-1], (R)q[0], (R)q[1], t0, up[v]);
-        recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], t1);
+        const double *q = a.P + v * ps + rb0;
+        recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
     }
     lo[0] = floor_max(fl, lo[0]);
     lo[1] = floor_max(fl, lo[1]);
     up[0] = floor_max(fl, up[0]);
     up[1] = floor_max(fl, up[1]);
-    R Ff[NV];
-    face_flux<NV, 0, RIEM>(up, lo, g, Ff);
+    R upl[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) out[v * vs] = (double)Ff[v];
+    for (int v = 0; v < NV; v++) upl[v] = shift_up1(up[v], up[v]);
+    R Ff[NV];
+    face_flux<NV, 0, RIEM>(upl, lo, g, Ff);
+    // the face needs the left cell's stencil in the grid too
+    const bool okl = ok && i != 0 && i != 4 && cell - 3 >= 0;
+    if (okl) {
+        const int e = i < 4 ? i - 1 : i - 2;
+        double *w = ebw + ((r - rb) & 7) * ebs + e * NV;
+#pragma unroll
+        for (int v = 0; v < NV; v++) w[v] = (double)Ff[v];
+    }
 }
 
 // ------------------------------------------------------------ y / z sweeps
@@ -340,14 +337,15 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     // lanes per block: 256 in 3D (four o-rows of 64 columns), 64 in 2D
     constexpr int WS = NV == 5 ? 256 : 64;
     // XY (3D y march only): the march also does the x direction of each row
-    // it updates, with the ring row of its 64 columns as the x stencils; the
-    // six faces at a wave's ends (three each side, k_xfaces) sit in a second
-    // ring [slot][wave][face][var].
+    // it updates, with the ring row of its 64 columns as the x stencils.  The
+    // six faces at a wave's ends whose stencils leave those columns (three
+    // each side) are solved every 8 rows for the next 8 by the whole wave
+    // (edge_faces) into eb[row & 7][wave][face][var].
     static_assert(!XY || (DIM == 1 && NV == 5 && !LAST), "x-y fusion is the 3D y march");
     constexpr int WSR = WS;  // ring stride per (slot, variable)
     __shared__ R win[5 * NV * WSR];
-    constexpr int BSL = (WS / 64) * 6 * NV;  // edge-face ring stride per slot
-    __shared__ double bring[XY ? 5 * BSL : 1];
+    constexpr int EBR = (WS / 64) * 6 * NV;  // edge-face buffer stride per row
+    __shared__ double eb[XY ? 8 * EBR : 1];
     // WENO-Z: the curvature term of each cell (wz_curv) is computed once and
     // carried: cell c needs the terms of cells c-1, c and c+1, and the term of
     // cell x lives in slot x & 1 of a two-slot ring [slot][var][lane].
@@ -396,31 +394,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                 wz_curv(wl[(1 * NV + v) * WSR], wl[(2 * NV + v) * WSR], wl[(3 * NV + v) * WSR]);
         }
     }
-    // XY edge faces of each ring row, one value per lane so that the
-    // prefetch costs one register: lane L < 30 carries variable L % 5 of
-    // edge face L / 5 = 0, 1, 2 (faces 2, 3, 4 of the wave's left edge
-    // b = blockIdx.x) or 3, 4, 5 (faces 0, 1, 2 of its right edge b + 1);
-    // interior rows only.  Lanes 0..2 and 62, 63 use them (the x part).
     const int wv = tid >> 6;
-    const int lb = min(lane, 29);  // lanes past 29 get in-range indices they never use
-    const int fe = lb / 5, fv = lb % 5;
-    const bool blane = XY && lane < 30;
-    // + j * nbf * 5 for row j
-    const double *bsrc = XY ? a.bface + ((long)fv * G.NZ + o) * G.NY * a.nbf * 5 +
-                                  (blockIdx.x + (fe < 3 ? 0 : 1)) * 5 + (fe < 3 ? fe + 2 : fe - 3)
-                            : nullptr;
-    double *bl = bring + (wv * 6 + fe) * NV + fv;
-    double xv = 0.0;
-    if constexpr (XY) {
-        if (blane) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int j = m0 - 3 + k;
-                bl[k * BSL] = (j >= 3 && j < G.NY - 3) ? bsrc[(long)j * a.nbf * 5] : 0.0;
-            }
-            xv = (m0 + 1 < G.NY - 3) ? bsrc[(long)(m0 + 1) * a.nbf * 5] : 0.0;
-        }
-    }
     // XY: the edge face that is this lane's left face (-1: solved in the
     // wave): lanes 0..2 -> edge faces 0..2; lanes 62, 63 -> the right edge's
     // face at their column (x0 + lane relative to the edge xr = min(x0 + 64,
@@ -444,7 +418,6 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
         for (int v = 0; v < NV; v++) wl[(s4 * NV + v) * WSR] = (R)nx[v];
         if constexpr (XY) {
-            if (blane) bl[s4 * BSL] = xv;
             // the x stencils read other lanes' ring words (written by this
             // wave); keep the compiler from moving the reads above the writes
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -454,9 +427,6 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
         if (c + 3 < Nd) {
 #pragma unroll
             for (int v = 0; v < NV; v++) nx[v] = ld_once(&P[v * ps + cb + (c + 3) * sd]);
-            if constexpr (XY) {
-                if (blane) xv = (c + 3 < G.NY - 3) ? bsrc[(long)(c + 3) * a.nbf * 5] : 0.0;
-            }
         }
         // conserved state of cell c-1 (updated at the end of this iteration):
         // issued now, consumed after the reconstruction and the Riemann solve
@@ -494,13 +464,21 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
             for (int v = 0; v < NV; v++) F[v] = (double)Ff[v];
             if (upd) {
                 if constexpr (XY) {
+                    // every 8 rows: the edge faces of rows c-1 .. c+6
+                    if (((c - 1 - m0) & 7) == 0) {
+                        edge_faces<NV, R, REC, RIEM>(a, G, eb + wv * 6 * NV, EBR, lane, x0, xr, o, c - 1,
+                                                     min(c + 7, m1), m0);
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
                     // x direction of row c-1, as the x sweep does it: each
                     // lane reconstructs its cell (stencil from the ring row),
                     // solves its left face with its left neighbour's upper
                     // state (DPP shift) and takes its right face from its
                     // right neighbour; the faces whose stencils leave the
-                    // wave's 64 columns come from k_xfaces.  Uniform control
-                    // flow: the shifts read every lane of the wave.
+                    // wave's 64 columns come from edge_faces.  Uniform
+                    // control flow: the shifts read every lane of the wave.
                     R lox[NV], upx[NV];
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
@@ -516,9 +494,9 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                     for (int v = 0; v < NV; v++) uplx[v] = shift_up1(upx[v], upx[v]);
                     R Fxf[NV];
                     face_flux<NV, 0, RIEM>(uplx, lox, g, Fxf);
-                    // edge faces of row c-1 (k_xfaces): each lane reads its
-                    // own (lfe) and edge face 5 (lane 63's right face)
-                    const double *be = bring + s1 * BSL + wv * 6 * NV;
+                    // edge faces of row c-1: each lane reads its own (lfe)
+                    // and edge face 5 (lane 63's right face)
+                    const double *be = eb + ((c - 1 - m0) & 7) * EBR + wv * 6 * NV;
                     const int lf = lfe < 0 ? 0 : lfe;
 #pragma unroll
                     for (int v = 0; v < NV; v++) {
@@ -633,30 +611,25 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
 {
     const Geo &G = a.geo;
     const int nxb = (G.NX - 6 + 63) / 64;
-    // 3D fast FP64 form: the x direction runs inside the y march (k_xfaces
-    // first solves the faces at the marches' 64-column wave edges), which
+    // 3D fast FP64 form: the x direction runs inside the y march, which
     // saves the x sweep's write and the y sweep's re-read of the
     // accumulated U and one read of P per stage; the same fluxes and
     // updates in the same order as the x and y sweeps.
 #if defined(VRN_FAST)
-    const bool fxy = NV == 5 && std::is_same<R, double>::value && a.fuse_xy && a.bface != nullptr;
+    const bool fxy = NV == 5 && std::is_same<R, double>::value && a.fuse_xy;
 #else
     const bool fxy = false;
 #endif
     if (fxy) {
         if constexpr (NV == 5 && std::is_same<R, double>::value) {
-            const long nf = 5L * a.nbf * (G.NY - 6) * (G.NZ - 6);
-            timer(tctx, 10, 0);
-            hipLaunchKernelGGL((k_xfaces<5, R, REC, RIEM>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, a);
-            timer(tctx, 10, 1);
             SweepArgs ay = a;
             ay.dx_dim = dxs[1];
             ay.dx_x = dxs[0];
             int nseg = (G.NY - 6 + a.seg_len - 1) / a.seg_len;
-            timer(tctx, 11, 0);
+            timer(tctx, 10, 0);
             hipLaunchKernelGGL((k_march<5, 1, R, REC, RIEM, false, true>), dim3(nxb, (G.NZ - 6 + 3) / 4, nseg),
                                dim3(256), 0, s, ay);
-            timer(tctx, 11, 1);
+            timer(tctx, 10, 1);
         }
     } else {
         SweepArgs ax = a;

@@ -55,13 +55,9 @@ struct SweepArgs {
     int seg_len;        // y / z sweeps: interior cells per marching segment
     int rows_per_wave;  // x sweep: rows streamed by one wave (xseg == 1)
     int xseg;           // x sweep: segments per row (> 1 only with rows_per_wave == 1)
-    // 3D fast FP64 form, x and y directions in one march: the five x faces
-    // around each 64-column wave boundary b = 0..nbf-1 of every row,
-    // [v][k][j][b][f], face f between cells xf - 1 and xf, xf = min(3 + 64 b,
-    // NX - 3) - 2 + f, and the x spacing for their flux differences
-    double *bface;
-    int nbf;
-    int fuse_xy;      // 0: separate x sweep (the same results)
+    // 3D fast FP64 form, x and y directions in one march (0: separate x
+    // sweep, the same results), and the x spacing of its flux differences
+    int fuse_xy;
     double dx_x;
 };
 
