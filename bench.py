@@ -569,9 +569,9 @@ def measured_traffic(kernel, n, precision):
 
 
 # The kernels of one RK stage of the fused step, by arithmetic form: the fast
-# FP64 form runs the x direction inside the y march (k_xfaces solves the x
-# faces at the marches' wave edges first), the other forms a separate x sweep.
-STAGE_KERNELS = {"fp64fast": ("xfaces", "xysweep", "zsweep_c2p"), "fp64": ("xsweep", "ysweep", "zsweep_c2p")}
+# FP64 form runs the x direction inside the y march, the other forms a
+# separate x sweep.
+STAGE_KERNELS = {"fp64fast": ("xysweep", "zsweep_c2p"), "fp64": ("xsweep", "ysweep", "zsweep_c2p")}
 
 
 def measured_step_traffic(n, precision):
