@@ -679,14 +679,22 @@ def main():
     tot = {k: c * m for k, (c, m) in rep.items()}
     dom = max(tot, key=tot.get)
     cnt, mean_ms = rep[dom]
-    # Algorithmic bytes per launch of the dominant kernel, per interior cell.
-    # The z sweep closes each RK stage, so it carries the stage's compulsory
-    # traffic: 560 B per zone-update over 3 stages (stage 1: read P, U^n, write
-    # U, P = 160 B; stages 2-3 also read U^n = 200 B; mean 560/3 B per launch).
-    # The x and y sweeps' own contract is read P, read U_b/acc, write acc =
-    # 120 B; so is the fused x-y march's (read P and U_b, write acc).
-    per_cell = {"zsweep_c2p": BYTES_PER_ZU["rk3"] / 3.0, "xsweep": 3 * 40, "ysweep": 3 * 40,
-                "xysweep": 3 * 40}.get(dom, 4 * 40)
+    # Algorithmic bytes per launch of the dominant kernel, per interior cell,
+    # averaged over the 3 RK stages.  Exact form: the z sweep closes each
+    # stage and forms the RK combination, so it carries the stage's
+    # compulsory traffic: 560 B per zone-update over 3 stages (stage 1: read
+    # P, U^n, write U, P = 160 B; stages 2-3 also read U^n = 200 B; mean
+    # 560/3 B); the x and y sweeps read P and U_b/acc and write acc (120 B).
+    # Fast form: the fused x-y march reads P and U_b, in stages 2-3 also U^n
+    # (it forms a U^n + b U_b), and writes acc (120 B + 2/3 x 40 B), plus P of
+    # the 3 cells either side of each 64-column wave that its edge-face
+    # stencils reach (6 cells x 40 B per 64 cells = 3.75 B); the z sweep
+    # reads P and acc and writes U and P (160 B).
+    if args.precision == "fp64fast":
+        per_cell = {"zsweep_c2p": 4 * 40, "xysweep": 3 * 40 + 2 * 40 / 3 + 6 * 40 / 64}.get(dom, 4 * 40)
+    else:
+        per_cell = {"zsweep_c2p": BYTES_PER_ZU["rk3"] / 3.0, "xsweep": 3 * 40,
+                    "ysweep": 3 * 40}.get(dom, 4 * 40)
     ach = per_cell * cells / (mean_ms * 1e-3) / 1e9
     step_gbs = BYTES_PER_ZU["rk3"] * value / 1e9
     step_traffic = measured_step_traffic(n, args.precision)

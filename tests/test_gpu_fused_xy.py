@@ -1,8 +1,10 @@
-"""The 3D fast FP64 form's x direction inside the y march (k_xfaces + the XY
-march) against the same form with a separate x sweep: every face the x
-sweep solves is solved by the march (interior faces in-wave, the faces at
-its 64-column wave edges by k_xfaces) with the same operations, and the
-updates are applied in the same order, so the two give the same bits.
+"""The 3D fast FP64 form's x direction inside the y march (the XY march)
+against the same form with a separate x sweep: every face the x sweep
+solves is solved by the march (interior faces in-wave, the faces at its
+64-column wave edges in batches of 8 rows, one edge cell per lane) with the
+same operations, both form the stage's RK combination a U0 + b U_b before
+the x update, and the updates are applied in the same order, so the two
+give the same bits.
 Grids with ragged x extents cover partial last waves, a last wave of one
 active lane and extents that are multiples of 64; the oracle comparison of
 the fused form itself is in test_gpu_parity3d.py (the fast form's default)."""

@@ -108,7 +108,7 @@ template <int NV> __device__ __forceinline__ bool all_finite(const double *p)
 // Combine, con2prim, Limit, rate for one updated cell of the last sweep.
 // A cell counts as unrecovered when its con2prim result is not finite (the
 // reference's NaN marker, or a NaN from a branch that reported convergence).
-template <int NV>
+template <int NV, bool COMBINED = false>
 __device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const double *acc,
                                             const double *u0, const double *guess, bool valid,
                                             double &rate)
@@ -116,7 +116,7 @@ __device__ __forceinline__ void finish_cell(const SweepArgs &a, long cu, const d
     const long ps = a.geo.ps;
     double out[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) out[v] = (a.a == 0.0) ? acc[v] : a.a * u0[v] + a.b * acc[v];
+    for (int v = 0; v < NV; v++) out[v] = (COMBINED || a.a == 0.0) ? acc[v] : a.a * u0[v] + a.b * acc[v];
     bool need = false;
     if (valid) {
 #pragma unroll
@@ -163,6 +163,18 @@ template <class R, int REC> constexpr int xsweep_waves() { return REC == REC_WEN
 template <class R, int REC> constexpr int xsweep_waves() { return 1; }
 #endif
 
+// Where the RK combination U' = a U0 + b (U_b + dt L) of a stage is formed.
+// Exact FP64 and FP32 forms and 2D: by the last sweep, from the accumulated
+// update (the reference's order).  3D fast FP64 form: by the first sweep of
+// the stage, as a U0 + b U_b, with every sweep's update scaled by b, so the
+// last sweep (the HBM-bound one) reads no U0.  Same value, different
+// rounding; the x sweep + y march and the fused x-y march do it alike.
+#if defined(VRN_FAST)
+template <int NV> constexpr bool rk_first() { return NV == 5; }
+#else
+template <int NV> constexpr bool rk_first() { return false; }
+#endif
+
 template <int NV, class R, int REC, int RIEM, bool LAST>
 __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepArgs a)
 {
@@ -191,7 +203,8 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
     const int total = (r1 - r0) * E;
     const long ps = G.ps;
     const R g = (R)a.g, fl = (R)a.floor_;
-    const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
+    constexpr bool PRE = rk_first<NV>() && !LAST;
+    const double cdt = (PRE ? a.b : 1.0) * (((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim);
     const double *__restrict__ P = a.P;
 
     R cup[NV];
@@ -216,9 +229,12 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
         const bool upd = act && c >= x0 + 1;
         const long cu = rbu + (c - 1);
         // base state of the cell this lane updates, issued before the flux work
-        double ub[NV];
+        double ub[NV], u0[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) ub[v] = upd ? a.Ub[v * ps + cu] : 0.0;
+        for (int v = 0; v < NV; v++) {
+            ub[v] = upd ? a.Ub[v * ps + cu] : 0.0;
+            u0[v] = (PRE && upd && a.a != 0.0) ? a.U0[v * ps + cu] : 0.0;
+        }
         R lo[NV], up[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -242,12 +258,16 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
         }
         if constexpr (!LAST) {
             if (upd) {
+                if (PRE && a.a != 0.0) {
+#pragma unroll
+                    for (int v = 0; v < NV; v++) ub[v] = a.a * u0[v] + a.b * ub[v];
+                }
 #pragma unroll
                 for (int v = 0; v < NV; v++) st_once(&a.Uacc[v * ps + cu], ub[v] - cdt * (F[v] - Fl[v]));
             }
         } else {
             // 2D: the x sweep is never last; kept for completeness of the template.
-            double acc[NV], u0[NV], guess[NV];
+            double acc[NV], guess[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
                 acc[v] = upd ? ub[v] - cdt * (F[v] - Fl[v]) : 0.0;
@@ -370,7 +390,10 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     const long cb = DIM == 1 ? gidx(G, xx, 0, NV == 5 ? o : 0) : gidx(G, xx, o, 0);
     const long ps = G.ps;
     const R g = (R)a.g, fl = (R)a.floor_;
-    const double cdt = ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim;
+    // 3D fast FP64 form: updates scaled by the RK weight b; the x-y march
+    // forms a U0 + b U_b itself, the z sweep reads no U0
+    constexpr bool PRE = rk_first<NV>();
+    const double cdt = (PRE ? a.b : 1.0) * (((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_dim);
     const double *__restrict__ P = a.P;
     // the x-y march starts from the stage's base state (there is no x sweep)
     const double *Ui = XY ? a.Ub : a.Uacc;
@@ -402,7 +425,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     const int x0 = 3 + blockIdx.x * 64;
     const int xr = min(x0 + 64, G.NX - 3);
     const int lfe = lane < 3 ? lane : (lane >= 62 ? 3 + min(max(x0 + lane - xr + 2, 0), 2) : -1);
-    const double cdtx = XY ? ((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_x : 0.0;
+    const double cdtx = XY ? (PRE ? a.b : 1.0) * (((a.dt_dev ? a.st->dt : a.dt_fixed) * a.dtfac) / a.dx_x) : 0.0;
     R upp[NV];
     double Fp[NV];
 #pragma unroll
@@ -436,7 +459,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             ub[v] = upd ? ld_once(&Ui[v * ps + cu]) : 0.0;
-            u0b[v] = (LAST && upd && a.a != 0.0) ? ld_once(&a.U0[v * ps + cu]) : 0.0;
+            u0b[v] = (((LAST && !PRE) || (XY && PRE)) && upd && a.a != 0.0) ? ld_once(&a.U0[v * ps + cu]) : 0.0;
         }
         R lo[NV], up[NV];
 #pragma unroll
@@ -464,6 +487,10 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
             for (int v = 0; v < NV; v++) F[v] = (double)Ff[v];
             if (upd) {
                 if constexpr (XY) {
+                    if (PRE && a.a != 0.0) {
+#pragma unroll
+                        for (int v = 0; v < NV; v++) ub[v] = a.a * u0b[v] + a.b * ub[v];
+                    }
                     // every 8 rows: the edge faces of rows c-1 .. c+6
                     if (((c - 1 - m0) & 7) == 0) {
                         edge_faces<NV, R, REC, RIEM>(a, G, eb + wv * 6 * NV, EBR, lane, x0, xr, o, c - 1,
@@ -524,7 +551,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                         else
                             guess[v] = P[v * ps + cu];
                     }
-                    finish_cell<NV>(a, cu, acc, u0b, guess, act, rate);
+                    finish_cell<NV, PRE>(a, cu, acc, u0b, guess, act, rate);
                 }
             }
 #pragma unroll
