@@ -61,14 +61,10 @@ struct GhostArgs {
     int nozzle;
     double noz_r2, noz_dx, noz_dy, noz_rho, noz_u, noz_ux, noz_uy, noz_uz;
     DevState *st;
+    // 3D: leave the x faces (the cells with an x ghost coordinate and
+    // interior y, z) unfilled; the next x-y march maps its x reads instead
+    int skip_x;
 };
-
-__device__ __forceinline__ int map_coord(int c, int N, int lo, int hi)
-{
-    if (c < 3) return lo == 1 ? c + (N - 6) : (lo == 2 ? 3 : c);
-    if (c >= N - 3) return hi == 1 ? c - (N - 6) : (hi == 2 ? N - 4 : c);
-    return c;
-}
 
 template <int NV>
 __global__ __launch_bounds__(256) void k_ghost(GhostArgs a)
@@ -521,7 +517,7 @@ template <int NV> void launch_ghost(vrn_ctx *c, GhostArgs &ga)
     const Geo &G = c->G;
     long n;
     if (NV == 5)
-        n = 6L * G.NX * G.NY + 6L * G.NX * (G.NZ - 6) + 6L * (G.NY - 6) * (G.NZ - 6);
+        n = 6L * G.NX * G.NY + 6L * G.NX * (G.NZ - 6) + (ga.skip_x ? 0L : 6L * (G.NY - 6) * (G.NZ - 6));
     else
         n = 6L * G.NX + 6L * (G.NY - 6);
     Timed tm(c, KN_GHOST);
@@ -1026,7 +1022,19 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     a.fuse_xy = c->fuse_xy;
     a.dx_x = p->dx;
     const int nst = p->integrator == VRN_INT_RK3 ? 3 : 2;
+    // stage-end ghost rules (the same every stage)
+    int rlo[3], rhi[3], rnoz;
+    make_rules(p->bc, p->ref_mode != 0, false, rlo, rhi, rnoz);
+    // 3D fast form with the fused x-y march: between stages the ghost pass
+    // skips the x faces when both have a rule, and the next x-y march (their
+    // only reader) maps its x reads through the rules; the last stage fills
+    // every face, so the state a step leaves is complete
+    const bool skip_x = p->precision == VRN_PREC_FP64_FAST && c->NV == 5 && c->fuse_xy && rlo[0] != 0 &&
+                        rhi[0] != 0;
+    bool x_unfilled = false;
     for (int s = 0; s < nst; s++) {
+        a.xmap_lo = x_unfilled ? rlo[0] : 0;
+        a.xmap_hi = x_unfilled ? rhi[0] : 0;
         a.P = c->P;
         a.Pout = c->Pn;
         a.do_rate = (cflm && s == nst - 1) ? 1 : 0;
@@ -1064,6 +1072,8 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
         ga.do_limit = 1;
         ga.floor_ = p->floor_;
         ga.st = c->st;
+        ga.skip_x = (skip_x && s < nst - 1) ? 1 : 0;
+        x_unfilled = ga.skip_x != 0;
         if (c->NV == 5) launch_ghost<5>(c, ga);
         else launch_ghost<4>(c, ga);
         HIPCHK(hipGetLastError());

@@ -14,6 +14,7 @@ using vrn::Geo;
 using vrn::DevState;
 using vrn::SweepArgs;
 using vrn::gidx;
+using vrn::map_coord;
 using vrn::XOFF;
 
 // ------------------------------------------------------------ wave helpers
@@ -174,6 +175,13 @@ template <int NV> constexpr bool rk_first() { return NV == 5; }
 #else
 template <int NV> constexpr bool rk_first() { return false; }
 #endif
+// a U0 + b U_b with one explicit rounding order (left to FMA contraction,
+// a x + b y may become either fma(a, x, b y) or fma(b, y, a x), and the
+// x sweep and the x-y march must round alike)
+__device__ __forceinline__ double rk_base(double a, double u0, double b, double ub)
+{
+    return __builtin_fma(a, u0, b * ub);
+}
 
 template <int NV, class R, int REC, int RIEM, bool LAST>
 __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepArgs a)
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
             if (upd) {
                 if (PRE && a.a != 0.0) {
 #pragma unroll
-                    for (int v = 0; v < NV; v++) ub[v] = a.a * u0[v] + a.b * ub[v];
+                    for (int v = 0; v < NV; v++) ub[v] = rk_base(a.a, u0[v], a.b, ub[v]);
                 }
 #pragma unroll
                 for (int v = 0; v < NV; v++) st_once(&a.Uacc[v * ps + cu], ub[v] - cdt * (F[v] - Fl[v]));
@@ -306,8 +314,11 @@ __global__ __launch_bounds__(256, (xsweep_waves<R, REC>())) void k_xsweep(SweepA
 // r0 + L / 8, as the x sweep does (both faces, floors), and lanes with i != 0, 4
 // solve the face with their left neighbour's upper state (DPP shift):
 // exactly the x sweep's faces.  Faces of rows outside [r0, r1) or whose
-// stencil leaves the grid are not needed and not written.  Out: ebw[(row -
-// rb) & 7 row stride ebs][face][var].
+// stencil leaves the grid are not needed and not written.  Stencil cells in
+// ghost columns are read through the x boundary rules (xmap_lo / xmap_hi)
+// when the last ghost pass left the x faces unfilled, which gives the values
+// that pass would have written.  Out: ebw[(row - rb) & 7 row stride
+// ebs][face][var].
 template <int NV, class R, int REC, int RIEM>
 __device__ __forceinline__ void edge_faces(const SweepArgs &a, const Geo &G, double *ebw, int ebs, int lane, int x0,
                                            int xr, int o, int r0, int r1, int rb)
@@ -320,12 +331,28 @@ __device__ __forceinline__ void edge_faces(const SweepArgs &a, const Geo &G, dou
     const long rb0 = gidx(G, cc, rr, o);
     const long ps = G.ps;
     const R g = (R)a.g, fl = (R)a.floor_;
+    // wave-uniform: only the waves at the x ends of the grid read ghost columns
+    const bool mapx = (a.xmap_lo | a.xmap_hi) != 0 && (x0 == 3 || xr == G.NX - 3);
+    R q5[5][NV];
+    if (mapx) {
+        const long row = rb0 - cc;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const long xs = row + map_coord(cc - 2 + k, G.NX, a.xmap_lo, a.xmap_hi);
+#pragma unroll
+            for (int v = 0; v < NV; v++) q5[k][v] = (R)a.P[v * ps + xs];
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double *q = a.P + v * ps + rb0;
+#pragma unroll
+            for (int k = 0; k < 5; k++) q5[k][v] = (R)q[k - 2];
+        }
+    }
     R lo[NV], up[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        const double *q = a.P + v * ps + rb0;
-        recon<REC>((R)q[-2], (R)q[-1], (R)q[0], (R)q[1], (R)q[2], lo[v], up[v]);
-    }
+    for (int v = 0; v < NV; v++) recon<REC>(q5[0][v], q5[1][v], q5[2][v], q5[3][v], q5[4][v], lo[v], up[v]);
     lo[0] = floor_max(fl, lo[0]);
     lo[1] = floor_max(fl, lo[1]);
     up[0] = floor_max(fl, up[0]);
@@ -383,8 +410,10 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
     if (o >= No - 3 || m0 >= Nd - 3) return;
     const bool act = x < G.NX - 3;
     // XY: every lane holds its own column (inactive lanes a ghost or the
-    // last one), since the x stencils of the active lanes reach into them
-    const int xx = XY ? min(x, G.NX - 1) : (act ? x : 3);
+    // last one), since the x stencils of the active lanes reach into them;
+    // ghost columns through the x boundary rules when the last ghost pass
+    // left the x faces unfilled
+    const int xx = XY ? map_coord(min(x, G.NX - 1), G.NX, a.xmap_lo, a.xmap_hi) : (act ? x : 3);
     const int m1 = min(m0 + a.seg_len, Nd - 3);
     const long sd = DIM == 1 ? (long)G.px : (long)G.px * G.NY;
     const long cb = DIM == 1 ? gidx(G, xx, 0, NV == 5 ? o : 0) : gidx(G, xx, o, 0);
@@ -489,7 +518,7 @@ __global__ __launch_bounds__(256, VRN_MARCH_MIN_WAVES) void k_march(SweepArgs a)
                 if constexpr (XY) {
                     if (PRE && a.a != 0.0) {
 #pragma unroll
-                        for (int v = 0; v < NV; v++) ub[v] = a.a * u0b[v] + a.b * ub[v];
+                        for (int v = 0; v < NV; v++) ub[v] = rk_base(a.a, u0b[v], a.b, ub[v]);
                     }
                     // every 8 rows: the edge faces of rows c-1 .. c+6
                     if (((c - 1 - m0) & 7) == 0) {

@@ -16,6 +16,16 @@ struct Geo {
     long ps;         // plane stride in doubles
 };
 
+// Boundary rule of one face applied to one coordinate c of an axis with N
+// cells (ghosts included): 0 identity, 1 periodic wrap, 2 zero-gradient copy.
+// Ghost coordinates map to the interior cell whose value the ghost holds.
+__host__ __device__ __forceinline__ int map_coord(int c, int N, int lo, int hi)
+{
+    if (c < 3) return lo == 1 ? c + (N - 6) : (lo == 2 ? 3 : c);
+    if (c >= N - 3) return hi == 1 ? c - (N - 6) : (hi == 2 ? N - 4 : c);
+    return c;
+}
+
 __host__ __device__ __forceinline__ long gidx(const Geo &G, int i, int j, int k)
 {
     return XOFF + i + (long)G.px * (j + (long)G.NY * k);
@@ -59,6 +69,10 @@ struct SweepArgs {
     // sweep, the same results), and the x spacing of its flux differences
     int fuse_xy;
     double dx_x;
+    // x-y march: boundary rules through which its x reads of ghost columns
+    // are mapped to interior cells (0, 0: read the ghost cells; used after
+    // a stage end whose ghost pass left the x faces unfilled)
+    int xmap_lo, xmap_hi;
 };
 
 // Batched 1D tubes (one persistent launch, one tube per workgroup at a time).
