@@ -62,3 +62,54 @@ def test_fused_xy_all_methods(rec, riemann):
     B, _, _ = _run(P, False, 2, rec=rec, riemann=riemann, integrator="rk2", ref=True,
                    bc=["periodic"] * 6)
     assert np.array_equal(interior(A), interior(B))
+
+
+def _jet(fused_seq, steps=6, n=40, nzc=32):
+    """The jet workload at small size (nozzle inflow on z-low with a per-step
+    precession angle, outflow elsewhere, HLLC + SSP-RK3 + CFL dt); step s runs
+    with the x-y fusion on or off as fused_seq[s] says.  Returns P with its
+    ghost cells and the dt of every step."""
+    import math
+
+    from verona_amd import DeviceGrid, host_array
+    from verona_amd.bc import uniform_stream
+    P = host_array(3, n, n, nzc)
+    P[0] = 1.0
+    P[1] = 1e-4 / (G - 1)
+    dx = 1 / n
+    it = uniform_stream(42)
+    phis = [2 * math.pi * next(it) for _ in range(steps)]
+    noz = (8 * dx, 0.01, 1e-4 / (G - 1), math.sqrt(99.0), 0.01)
+    bc = ["outflow"] * 4 + ["nozzle", "outflow"]
+    with DeviceGrid(3, n, n, nzc, G) as dev:
+        dev.upload("P", P)
+        dev.set_nozzle(noz[0], dx, dx, *noz[1:])
+        dev.fill_ghosts(bc)
+        dev.prim_to_cons()
+        dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision="fp64fast",
+                     n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dx, dz=dx, bc=bc)
+        ts = []
+        for s in range(steps):
+            dev.set_xy_fusion(fused_seq[s])
+            ts.append(dev.step(phi=phis[s], want_dt=True))
+        out = dev.download("P")
+        nb = dev.counters()[0]
+    return out, ts, nb
+
+
+def test_fused_xy_jet_nozzle_ghost_cells_read_through_the_rules():
+    """With the fused march and rules on both x faces the stage-end ghost
+    passes leave the x faces unfilled, across steps too: the fused march reads
+    ghost columns through the x rules.  On the jet (nozzle + outflow) this
+    equals the separate sweeps with complete ghost passes, bit for bit, ghost
+    cells included (the download fills the x faces first); so does switching
+    to the separate sweeps half-way, which fills them before its first stage."""
+    progress("test_fused_xy_jet_nozzle_ghost_cells_read_through_the_rules 40x40x32")
+    A, ta, na = _jet([True] * 6)
+    B, tb, nbb = _jet([False] * 6)
+    C, tc, nc = _jet([True, True, True, False, False, True])
+    progress(f"  max |fused - separate| {np.abs(A - B).max():.3e}, max u_z {A[4].max():.3f}")
+    assert na == nbb == nc == 0
+    assert ta == tb == tc
+    assert np.array_equal(A, B) and np.array_equal(C, B)
+    assert A[4, 3:-3, 3:-3, 3].max() > 0.5  # the jet has entered the grid

@@ -61,16 +61,17 @@ struct GhostArgs {
     int nozzle;
     double noz_r2, noz_dx, noz_dy, noz_rho, noz_u, noz_ux, noz_uy, noz_uz;
     DevState *st;
-    // 3D: leave the x faces (the cells with an x ghost coordinate and
-    // interior y, z) unfilled; the next x-y march maps its x reads instead
-    int skip_x;
+    // 3D cells filled: 0 every ghost cell, 1 all but the x faces (cells with
+    // an x ghost coordinate and interior y and z), 2 the x faces only
+    int part;
 };
 
 template <int NV>
 __global__ __launch_bounds__(256) void k_ghost(GhostArgs a)
 {
     const Geo G = a.geo;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (NV == 5 && a.part == 2) t += 6L * G.NX * G.NY + 6L * G.NX * (G.NZ - 6);
     if (t == 0 && a.st != nullptr) {
         a.st->n_rescued += a.st->rescue_count;
         a.st->rescue_count = 0;
@@ -396,6 +397,12 @@ struct vrn_ctx {
     double *buf = nullptr;   // P, Pn, U, Uh and the staging buffer
     double *fbuf = nullptr;  // flux slots, allocated by the first per-kernel call that uses them
     int fuse_xy = 1;  // 3D fast FP64 form: x direction inside the y march
+    // 3D fast FP64 form with the fused x-y march and rules on both x faces:
+    // the stage-end ghost passes leave the x faces unfilled (the x-y march,
+    // their only reader in the step, reads ghost columns through the rules);
+    // `pend` fills them (settle) before anything else reads P's ghost cells
+    bool ghosts_pending = false;
+    vrn::GhostArgs pend{};
     double *P = nullptr, *Pn = nullptr, *U = nullptr, *Uh = nullptr, *F[3] = {nullptr, nullptr, nullptr};
     double *stg = nullptr;
     int *rescue = nullptr;
@@ -517,7 +524,8 @@ template <int NV> void launch_ghost(vrn_ctx *c, GhostArgs &ga)
     const Geo &G = c->G;
     long n;
     if (NV == 5)
-        n = 6L * G.NX * G.NY + 6L * G.NX * (G.NZ - 6) + (ga.skip_x ? 0L : 6L * (G.NY - 6) * (G.NZ - 6));
+        n = (ga.part == 2 ? 0L : 6L * G.NX * G.NY + 6L * G.NX * (G.NZ - 6)) +
+            (ga.part == 1 ? 0L : 6L * (G.NY - 6) * (G.NZ - 6));
     else
         n = 6L * G.NX + 6L * (G.NY - 6);
     Timed tm(c, KN_GHOST);
@@ -569,6 +577,20 @@ int check_ctx(vrn_ctx *c)
 // one row; y / z sweeps: shorter marching segments when columns are few).
 // The partition changes no arithmetic: every face is solved once and every
 // cell updated once from the same operands.
+// Fill the x faces the last stage-end ghost pass left unfilled, if any.
+static int settle(vrn_ctx *c)
+{
+    if (!c->ghosts_pending) return 0;
+    c->ghosts_pending = false;
+    c->pend.Pd = c->P;
+    c->pend.Pi = c->P;
+    c->pend.Pg = c->P;
+    c->pend.part = 2;
+    launch_ghost<5>(c, c->pend);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static void partition_sweeps(vrn_ctx *c)
 {
     const int Nxi = c->G.NX - 6, Nyi = c->G.NY - 6, Nzi = c->ndim == 3 ? c->G.NZ - 6 : 1;
@@ -699,7 +721,10 @@ int vrn_upload(vrn_ctx *c, int slot, const double *host)
     else
         hipLaunchKernelGGL(k_aos_to_soa<4>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, G, c->stg, d);
     HIPCHK(hipGetLastError());
-    if (slot == VRN_SLOT_P) c->rate_ok = false;
+    if (slot == VRN_SLOT_P) {
+        c->rate_ok = false;
+        c->ghosts_pending = false;
+    }
     HIPCHK(hipStreamSynchronize(c->s));
     return 0;
 }
@@ -711,6 +736,8 @@ int vrn_download(vrn_ctx *c, int slot, double *host)
         if (int r = ensure_flux(c)) return r;
     double *d = slot_ptr(c, slot);
     if (!d || !host) return fail(VRN_E_ARG, "bad slot %d or null host pointer", slot);
+    if (slot == VRN_SLOT_P)
+        if (int r = settle(c)) return r;
     const Geo &G = c->G;
     long n = (long)G.NX * G.NY * G.NZ;
     if (c->NV == 5)
@@ -730,8 +757,13 @@ int vrn_copy(vrn_ctx *c, int dst_slot, int src_slot)
         if (int r = ensure_flux(c)) return r;
     double *d = slot_ptr(c, dst_slot), *s_ = slot_ptr(c, src_slot);
     if (!d || !s_ || d == s_) return fail(VRN_E_ARG, "bad slots %d <- %d", dst_slot, src_slot);
+    if (src_slot == VRN_SLOT_P)
+        if (int r = settle(c)) return r;
     HIPCHK(hipMemcpyAsync(d, s_, sizeof(double) * (size_t)c->NV * c->G.ps, hipMemcpyDeviceToDevice, c->s));
-    if (dst_slot == VRN_SLOT_P) c->rate_ok = false;
+    if (dst_slot == VRN_SLOT_P) {
+        c->rate_ok = false;
+        c->ghosts_pending = false;
+    }
     return 0;
 }
 
@@ -745,6 +777,7 @@ int vrn_slot_bytes(vrn_ctx *c, int64_t *bytes)
 int vrn_prim_to_cons(vrn_ctx *c)
 {
     if (int r = check_ctx(c)) return r;
+    if (int r = settle(c)) return r;
     long n = (long)c->G.NX * c->G.NY * c->G.NZ;
     if (c->NV == 5)
         hipLaunchKernelGGL(k_ref_ptou<5>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, c->G, c->P, c->U, c->gamma);
@@ -757,6 +790,7 @@ int vrn_prim_to_cons(vrn_ctx *c)
 int vrn_limit(vrn_ctx *c, double floor_)
 {
     if (int r = check_ctx(c)) return r;
+    if (int r = settle(c)) return r;
     long n = (long)c->G.NX * c->G.NY * c->G.NZ;
     if (c->NV == 5)
         hipLaunchKernelGGL(k_ref_limit<5>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->s, c->G, c->P, floor_);
@@ -797,6 +831,7 @@ int vrn_fluxes(vrn_ctx *c, int dim, double floor_, int rec, int riemann)
     if (dim < 0 || dim >= c->ndim) return fail(VRN_E_ARG, "dim %d out of range", dim);
     if (rec < 0 || rec > 2 || riemann < 0 || riemann > 1) return fail(VRN_E_ARG, "bad rec/riemann");
     if (int r = ensure_flux(c)) return r;
+    if (int r = settle(c)) return r;
     Timed tm(c, KN_FLUX);
     if (c->NV == 5) {
         if (dim == 0) launch_ref_flux<5, 0>(c, rec, riemann, floor_);
@@ -832,6 +867,7 @@ int vrn_cons_to_prim(vrn_ctx *c, int src_slot, int n_iter, double tol)
     if (int r = check_ctx(c)) return r;
     if (src_slot != VRN_SLOT_U && src_slot != VRN_SLOT_UHALF) return fail(VRN_E_ARG, "src must be U or UHALF");
     if (n_iter < 0) return fail(VRN_E_ARG, "n_iter must be >= 0");
+    if (int r = settle(c)) return r;
     const Geo &G = c->G;
     long n = (long)(G.NX - 6) * (G.NY - 6) * (c->NV == 5 ? G.NZ - 6 : 1);
     Timed tm(c, KN_C2P);
@@ -886,6 +922,7 @@ int vrn_fill_ghosts(vrn_ctx *c, const int bc[6])
     for (int f = 0; f < 6; f++)
         if (bc[f] < 0 || bc[f] > 3 || (bc[f] == VRN_BC_NOZZLE && f != 4))
             return fail(VRN_E_ARG, "bad boundary kind %d on face %d", bc[f], f);
+    if (int r = settle(c)) return r;
     return fill_ghosts_phi(c, bc, 0.0, false, false);
 }
 
@@ -967,6 +1004,16 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
             return fail(VRN_E_ARG, "bad boundary kind %d on face %d (nozzle needs vrn_set_nozzle)", p->bc[f], f);
     const Geo &G = c->G;
     const bool cflm = p->cfl > 0.0;
+    // stage-end ghost rules (the same every stage)
+    int rlo[3], rhi[3], rnoz;
+    make_rules(p->bc, p->ref_mode != 0, false, rlo, rhi, rnoz);
+    // 3D fast FP64 form with the fused x-y march and a rule on both x faces
+    // (so each x ghost cell is a copy of an interior cell): the stage-end
+    // ghost passes leave the x faces unfilled and the x-y march reads ghost
+    // columns through the rules.  Anything else first fills them.
+    const bool defer = p->precision == VRN_PREC_FP64_FAST && c->NV == 5 && c->fuse_xy && rlo[0] && rhi[0];
+    if (!defer)
+        if (int r = settle(c)) return r;
     // dt: fixed, or CFL from the state at the step start
     const double dz_rate = c->ndim == 3 ? p->dz : 1.0;
     if (cflm) {
@@ -1022,19 +1069,10 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     a.fuse_xy = c->fuse_xy;
     a.dx_x = p->dx;
     const int nst = p->integrator == VRN_INT_RK3 ? 3 : 2;
-    // stage-end ghost rules (the same every stage)
-    int rlo[3], rhi[3], rnoz;
-    make_rules(p->bc, p->ref_mode != 0, false, rlo, rhi, rnoz);
-    // 3D fast form with the fused x-y march: between stages the ghost pass
-    // skips the x faces when both have a rule, and the next x-y march (their
-    // only reader) maps its x reads through the rules; the last stage fills
-    // every face, so the state a step leaves is complete
-    const bool skip_x = p->precision == VRN_PREC_FP64_FAST && c->NV == 5 && c->fuse_xy && rlo[0] != 0 &&
-                        rhi[0] != 0;
-    bool x_unfilled = false;
     for (int s = 0; s < nst; s++) {
-        a.xmap_lo = x_unfilled ? rlo[0] : 0;
-        a.xmap_hi = x_unfilled ? rhi[0] : 0;
+        // x ghost columns through the rules of the pass that left them unfilled
+        a.xmap_lo = defer && c->ghosts_pending ? c->pend.rule_lo[0] : 0;
+        a.xmap_hi = defer && c->ghosts_pending ? c->pend.rule_hi[0] : 0;
         a.P = c->P;
         a.Pout = c->Pn;
         a.do_rate = (cflm && s == nst - 1) ? 1 : 0;
@@ -1072,10 +1110,13 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
         ga.do_limit = 1;
         ga.floor_ = p->floor_;
         ga.st = c->st;
-        ga.skip_x = (skip_x && s < nst - 1) ? 1 : 0;
-        x_unfilled = ga.skip_x != 0;
+        ga.part = defer ? 1 : 0;
         if (c->NV == 5) launch_ghost<5>(c, ga);
         else launch_ghost<4>(c, ga);
+        if (defer) {
+            c->pend = ga;
+            c->ghosts_pending = true;
+        }
         HIPCHK(hipGetLastError());
         std::swap(c->P, c->Pn);
     }
