@@ -700,6 +700,11 @@ def main():
     step_traffic = measured_step_traffic(n, args.precision)
     ceiling = valu_issue_ceiling(n, args.precision)
     copy_gbs = copy_bandwidth(n)
+    fp64v = fp64_roofline(dom, cells, mean_ms, n, args.precision)
+    # which of the two the dominant kernel sits closer to: its VALU issue
+    # fraction (instruction-mix profile) or its HBM rate over the device copy
+    limited_by = ("valu_issue" if fp64v is not None and copy_gbs and fp64v["valu_issue_frac"] > ach / copy_gbs
+                  else "hbm")
     out = {
         "metric": "zone-updates/s (3D WENOZ+HLLC, 1 GPU); con2prim cells/s learned vs Newton",
         "value": value,
@@ -723,6 +728,8 @@ def main():
                      "measured_copy_gbs": copy_gbs, "frac_of_copy": ach / copy_gbs,
                      "traffic": measured_traffic(dom, n, args.precision),
                      "traffic_source": "profiles/jet384_rk3_hbm_traffic.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)",
+                     "limited_by": limited_by,
+                     "valu_issue_frac": None if fp64v is None else fp64v["valu_issue_frac"],
                      "bytes_per_cell_per_launch": per_cell, "mean_launch_ms": mean_ms,
                      "launches": cnt, "algorithmic_bytes": per_cell * cells,
                      # the metric itself: the whole step at 560 B per zone-update
@@ -733,7 +740,7 @@ def main():
                                                            else step_traffic / BYTES_PER_ZU["rk3"]),
                               "hbm_ceiling_zone_updates_per_s": HBM_PEAK_GBS * 1e9 / BYTES_PER_ZU["rk3"],
                               "valu_issue_ceiling": ceiling}},
-        "roofline_fp64": fp64_roofline(dom, cells, mean_ms, n, args.precision),
+        "roofline_fp64": fp64v,
         "roofline_step": {"bytes_per_zone_update": BYTES_PER_ZU["rk3"], "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
                           "measured_traffic_bytes_per_zone_update": step_traffic,
