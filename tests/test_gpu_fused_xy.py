@@ -113,3 +113,39 @@ def test_fused_xy_jet_nozzle_ghost_cells_read_through_the_rules():
     assert ta == tb == tc
     assert np.array_equal(A, B) and np.array_equal(C, B)
     assert A[4, 3:-3, 3:-3, 3].max() > 0.5  # the jet has entered the grid
+
+
+@pytest.mark.parametrize("entry", ["fluxes", "copy", "fill_ghosts", "limit"])
+def test_entry_points_after_fused_steps_see_complete_ghosts(entry):
+    """After fused fast-form steps that left P's x faces unfilled, each entry
+    point that reads P's ghost cells fills them first: its result equals the
+    same call after the separate sweeps (which fill every face), bit for bit."""
+    progress(f"test_entry_points_after_fused_steps_see_complete_ghosts {entry}")
+    from verona_amd import DeviceGrid
+    P = state3(70, 20, 16, seed=23)
+    nx, ny, nz = 70, 20, 16
+    bc = ["outflow", "outflow", "periodic", "periodic", "outflow", "outflow"]
+    out = []
+    for fused in (True, False):
+        with DeviceGrid(3, nx, ny, nz, G) as dev:
+            dev.set_xy_fusion(fused)
+            dev.upload("P", P)
+            dev.fill_ghosts(bc)
+            dev.prim_to_cons()
+            dev.set_step(ref_mode=False, rec="wenoz", riemann="hllc", integrator="rk3", precision="fp64fast",
+                         n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=1 / nx, dy=1 / ny, dz=1 / nz, bc=bc)
+            for _ in range(2):
+                dev.step()
+            if entry == "fluxes":
+                dev.fluxes(0, 1e-7)
+                out.append(dev.download("Fx"))
+            elif entry == "copy":
+                dev.copy("Uhalf", "P")
+                out.append(dev.download("Uhalf"))
+            elif entry == "fill_ghosts":
+                dev.fill_ghosts(["none"] * 2 + bc[2:])  # x faces kept as they are
+                out.append(dev.download("P"))
+            else:
+                dev.limit(1e-7)
+                out.append(dev.download("P"))
+    assert np.array_equal(out[0], out[1])
