@@ -568,10 +568,9 @@ def measured_traffic(kernel, n, precision):
     return None if k is None else float(k["bytes"])
 
 
-# The kernels of one RK stage of the fused step, by arithmetic form: the fast
-# FP64 form runs the x direction inside the y march, the other forms a
-# separate x sweep.
-STAGE_KERNELS = {"fp64fast": ("xysweep", "zsweep_c2p"), "fp64": ("xsweep", "ysweep", "zsweep_c2p")}
+# The kernels of one RK stage of the fused step, by arithmetic form: both
+# FP64 forms run the x direction inside the y march.
+STAGE_KERNELS = {"fp64fast": ("xysweep", "zsweep_c2p"), "fp64": ("xysweep", "zsweep_c2p")}
 
 
 def measured_step_traffic(n, precision):
@@ -684,7 +683,8 @@ def main():
     # stage and forms the RK combination, so it carries the stage's
     # compulsory traffic: 560 B per zone-update over 3 stages (stage 1: read
     # P, U^n, write U, P = 160 B; stages 2-3 also read U^n = 200 B; mean
-    # 560/3 B); the x and y sweeps read P and U_b/acc and write acc (120 B).
+    # 560/3 B); its x-y march (or the x and y sweeps) read P and U_b/acc and
+    # write acc (120 B, plus the edge stencils' 3.75 B).
     # Fast form: the fused x-y march reads P and U_b, in stages 2-3 also U^n
     # (it forms a U^n + b U_b), and writes acc (120 B + 2/3 x 40 B), plus P of
     # the 3 cells either side of each 64-column wave that its edge-face
@@ -693,8 +693,8 @@ def main():
     if args.precision == "fp64fast":
         per_cell = {"zsweep_c2p": 4 * 40, "xysweep": 3 * 40 + 2 * 40 / 3 + 6 * 40 / 64}.get(dom, 4 * 40)
     else:
-        per_cell = {"zsweep_c2p": BYTES_PER_ZU["rk3"] / 3.0, "xsweep": 3 * 40,
-                    "ysweep": 3 * 40}.get(dom, 4 * 40)
+        per_cell = {"zsweep_c2p": BYTES_PER_ZU["rk3"] / 3.0, "xsweep": 3 * 40, "ysweep": 3 * 40,
+                    "xysweep": 3 * 40 + 6 * 40 / 64}.get(dom, 4 * 40)
     ach = per_cell * cells / (mean_ms * 1e-3) / 1e9
     step_gbs = BYTES_PER_ZU["rk3"] * value / 1e9
     step_traffic = measured_step_traffic(n, args.precision)

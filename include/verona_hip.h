@@ -163,8 +163,8 @@ typedef struct {
 
 /* One full time step on the context's P (src/3dim/algos.jl:274-329):
  * per stage one fused kernel per direction (the last one also runs
- * con2prim + Limit; 3D fast FP64 form: x and y in one kernel), then the
- * ghost fill.  In the 3D fast FP64 form with the x-y fusion and a rule on
+ * con2prim + Limit; 3D FP64 forms: x and y in one kernel), then the
+ * ghost fill.  In the 3D FP64 forms with the x-y fusion and a rule on
  * both x faces the ghost fill leaves P's x faces to be filled by the next
  * entry point that reads P's ghost cells (the fused kernel reads them
  * through the rules), so callers always see complete ghost cells.
@@ -189,12 +189,12 @@ int vrn_timing_report(vrn_ctx *ctx, char *names, int name_len, int64_t *counts, 
 int vrn_set_partition(vrn_ctx *ctx, int rows_per_wave, int xseg, int seg_len);
 int vrn_get_partition(vrn_ctx *ctx, int *rows_per_wave, int *xseg, int *seg_len);
 
-/* 3D, VRN_PREC_FP64_FAST: whether the x direction runs inside the y march
- * (on = 1, the default) or as its own sweep (0).  No counterpart in the
- * reference and no change of arithmetic: both solve the same faces and apply
- * the same updates in the same order, bit for bit; the fused form moves
- * 116 B less per cell and stage.  Other forms and 2D always use the separate
- * x sweep. */
+/* 3D, VRN_PREC_FP64 and VRN_PREC_FP64_FAST: whether the x direction runs
+ * inside the y march (on = 1, the default) or as its own sweep (0).  No
+ * counterpart in the reference and no change of arithmetic: both solve the
+ * same faces and apply the same updates in the same order, bit for bit; the
+ * fused form moves 116 B less per cell and stage.  The FP32 form and 2D
+ * always use the separate x sweep. */
 int vrn_set_xy_fusion(vrn_ctx *ctx, int on);
 
 

@@ -1,13 +1,14 @@
-"""The 3D fast FP64 form's x direction inside the y march (the XY march)
-against the same form with a separate x sweep: every face the x sweep
+"""The 3D FP64 forms' x direction inside the y march (the XY march) against
+the same form with a separate x sweep: every face the x sweep
 solves is solved by the march (interior faces in-wave, the faces at its
 64-column wave edges in batches of 8 rows, one edge cell per lane) with the
-same operations, both form the stage's RK combination a U0 + b U_b before
-the x update, and the updates are applied in the same order, so the two
-give the same bits.
+same operations, the fast form forms the stage's RK combination a U0 + b U_b
+before the x update in both (the exact form in the z sweep in both), and the
+updates are applied in the same order, so the two give the same bits.
 Grids with ragged x extents cover partial last waves, a last wave of one
 active lane and extents that are multiples of 64; the oracle comparison of
-the fused form itself is in test_gpu_parity3d.py (the fast form's default)."""
+the fused form itself is in test_gpu_parity3d.py (the default of both FP64
+forms)."""
 import numpy as np
 import pytest
 
@@ -17,7 +18,8 @@ pytestmark = pytest.mark.gpu
 G = 4.0 / 3.0
 
 
-def _run(P, fused, steps, *, rec="wenoz", riemann="hllc", integrator="rk3", ref=False, bc=None):
+def _run(P, fused, steps, *, rec="wenoz", riemann="hllc", integrator="rk3", ref=False, bc=None,
+         precision="fp64fast"):
     from verona_amd import DeviceGrid
     nx, ny, nz = (s - 6 for s in P.shape[1:])
     dx, dy, dz = 1 / nx, 1 / ny, 1 / nz
@@ -29,10 +31,10 @@ def _run(P, fused, steps, *, rec="wenoz", riemann="hllc", integrator="rk3", ref=
         dev.prim_to_cons()
         if ref:
             dt = 0.4 / (1 / dx + 1 / dy + 1 / dz)
-            dev.set_step(ref_mode=True, rec=rec, riemann=riemann, integrator=integrator, precision="fp64fast",
+            dev.set_step(ref_mode=True, rec=rec, riemann=riemann, integrator=integrator, precision=precision,
                          n_iter=50, tol=1e-10, floor=1e-7, dt=dt, dx=dx, dy=dy, dz=dz, bc=bc)
         else:
-            dev.set_step(ref_mode=False, rec=rec, riemann=riemann, integrator=integrator, precision="fp64fast",
+            dev.set_step(ref_mode=False, rec=rec, riemann=riemann, integrator=integrator, precision=precision,
                          n_iter=50, tol=1e-10, floor=1e-7, cfl=0.4, dx=dx, dy=dy, dz=dz, bc=bc)
         ts = [dev.step(want_dt=not ref) for _ in range(steps)]
         out = dev.download("P")
@@ -40,12 +42,15 @@ def _run(P, fused, steps, *, rec="wenoz", riemann="hllc", integrator="rk3", ref=
     return out, ts, nb
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(40, 24, 20), (65, 20, 18), (128, 16, 12), (70, 33, 17), (64, 9, 11)])
-def test_fused_xy_equals_separate_x_sweep(nx, ny, nz):
-    progress(f"test_fused_xy_equals_separate_x_sweep {nx}x{ny}x{nz}")
+@pytest.mark.parametrize("nx,ny,nz,precision", [(40, 24, 20, "fp64fast"), (65, 20, 18, "fp64fast"),
+                                                (128, 16, 12, "fp64fast"), (70, 33, 17, "fp64fast"),
+                                                (64, 9, 11, "fp64fast"), (70, 33, 17, "fp64"),
+                                                (40, 24, 20, "fp64")])
+def test_fused_xy_equals_separate_x_sweep(nx, ny, nz, precision):
+    progress(f"test_fused_xy_equals_separate_x_sweep {nx}x{ny}x{nz} {precision}")
     P = state3(nx, ny, nz, seed=21)
-    A, ta, na = _run(P, True, 3)
-    B, tb, nbb = _run(P, False, 3)
+    A, ta, na = _run(P, True, 3, precision=precision)
+    B, tb, nbb = _run(P, False, 3, precision=precision)
     diff = np.abs(interior(A) - interior(B)).max()
     progress(f"  max |fused - separate| {diff:.3e}, dt {ta[-1]:.6e} / {tb[-1]:.6e}")
     assert na == nbb == 0

@@ -396,8 +396,8 @@ struct vrn_ctx {
     double gamma = 4.0 / 3.0;
     double *buf = nullptr;   // P, Pn, U, Uh and the staging buffer
     double *fbuf = nullptr;  // flux slots, allocated by the first per-kernel call that uses them
-    int fuse_xy = 1;  // 3D fast FP64 form: x direction inside the y march
-    // 3D fast FP64 form with the fused x-y march and rules on both x faces:
+    int fuse_xy = 1;  // 3D FP64 forms: x direction inside the y march
+    // 3D FP64 forms with the fused x-y march and rules on both x faces:
     // the stage-end ghost passes leave the x faces unfilled (the x-y march,
     // their only reader in the step, reads ghost columns through the rules);
     // `pend` fills them (settle) before anything else reads P's ghost cells
@@ -1007,11 +1007,12 @@ int vrn_step(vrn_ctx *c, const vrn_step_params *p, double *dt_used)
     // stage-end ghost rules (the same every stage)
     int rlo[3], rhi[3], rnoz;
     make_rules(p->bc, p->ref_mode != 0, false, rlo, rhi, rnoz);
-    // 3D fast FP64 form with the fused x-y march and a rule on both x faces
+    // 3D FP64 forms with the fused x-y march and a rule on both x faces
     // (so each x ghost cell is a copy of an interior cell): the stage-end
     // ghost passes leave the x faces unfilled and the x-y march reads ghost
     // columns through the rules.  Anything else first fills them.
-    const bool defer = p->precision == VRN_PREC_FP64_FAST && c->NV == 5 && c->fuse_xy && rlo[0] && rhi[0];
+    const bool defer = (p->precision == VRN_PREC_FP64_FAST || p->precision == VRN_PREC_FP64) && c->NV == 5 &&
+                       c->fuse_xy && rlo[0] && rhi[0];
     if (!defer)
         if (int r = settle(c)) return r;
     // dt: fixed, or CFL from the state at the step start

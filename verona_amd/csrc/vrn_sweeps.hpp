@@ -667,15 +667,11 @@ void launch_stage(hipStream_t s, const SweepArgs &a, const double dxs[3], void (
 {
     const Geo &G = a.geo;
     const int nxb = (G.NX - 6 + 63) / 64;
-    // 3D fast FP64 form: the x direction runs inside the y march, which
-    // saves the x sweep's write and the y sweep's re-read of the
-    // accumulated U and one read of P per stage; the same fluxes and
-    // updates in the same order as the x and y sweeps.
-#if defined(VRN_FAST)
+    // 3D FP64 forms: the x direction runs inside the y march, which saves
+    // the x sweep's write and the y sweep's re-read of the accumulated U and
+    // one read of P per stage; the same fluxes and updates in the same
+    // order as the x and y sweeps.
     const bool fxy = NV == 5 && std::is_same<R, double>::value && a.fuse_xy;
-#else
-    const bool fxy = false;
-#endif
     if (fxy) {
         if constexpr (NV == 5 && std::is_same<R, double>::value) {
             SweepArgs ay = a;
