@@ -172,10 +172,12 @@ typedef struct {
  * small device-to-host read. */
 int vrn_step(vrn_ctx *ctx, const vrn_step_params *p, double *dt_used);
 
-/* Per-kernel timing with HIP events on the context's stream.  While enabled,
- * every launch of the fused path is bracketed by events; vrn_timing_report
- * fills up to `cap` entries with the kernel name, launch count and mean
- * duration in milliseconds.  Blocks. */
+/* Per-kernel timing with HIP events on the context's stream.  enable = 1:
+ * every launch of the fused path is bracketed by events; 2: only the
+ * directional sweeps' launches (the x, y, x-y and z kernels), which puts
+ * fewer event records into the stream; 0: off.  vrn_timing_report fills up
+ * to `cap` entries with the kernel name, launch count and mean duration in
+ * milliseconds.  Blocks. */
 int vrn_timing_enable(vrn_ctx *ctx, int enable);
 int vrn_timing_report(vrn_ctx *ctx, char *names, int name_len, int64_t *counts, double *mean_ms,
                       int cap, int *n_out);

@@ -424,7 +424,7 @@ struct vrn_ctx {
     int seg_len = 96;
     int xseg = 1;
     // timing
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every kernel, 2 the directional sweeps only
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
     double t_ms[KN_COUNT] = {0};
@@ -454,7 +454,10 @@ struct Timed {
     int kn;
     std::pair<hipEvent_t, hipEvent_t> ev{};
     bool on;
-    Timed(vrn_ctx *c_, int kn_) : c(c_), kn(kn_), on(c_->timing)
+    Timed(vrn_ctx *c_, int kn_)
+        : c(c_), kn(kn_),
+          on(c_->timing == 1 ||
+             (c_->timing == 2 && (kn_ == KN_X || kn_ == KN_Y || kn_ == KN_ZC || kn_ == KN_XY)))
     {
         if (!on) return;
         if (c->ev_free.empty()) {
@@ -1176,7 +1179,8 @@ int vrn_timing_enable(vrn_ctx *c, int enable)
 {
     if (int r = check_ctx(c)) return r;
     if (int r = harvest(c)) return r;
-    c->timing = enable != 0;
+    if (enable < 0 || enable > 2) return fail(VRN_E_ARG, "timing mode %d (0 off, 1 all kernels, 2 sweeps)", enable);
+    c->timing = enable;
     for (int k = 0; k < KN_COUNT; k++) {
         c->t_ms[k] = 0;
         c->t_n[k] = 0;

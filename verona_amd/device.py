@@ -155,6 +155,7 @@ class DeviceGrid:
 
     # ------------------------------------------------------------------ timing
     def timing(self, enable=True):
+        # True / 1: every kernel; 2: the directional sweeps only
         check(self.lib.vrn_timing_enable(self.h, int(enable)), "vrn_timing_enable")
 
     def timing_report(self):
